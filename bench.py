@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""
+Benchmark: cell-steps/s of the 2D 1024 x 1024 Luo-Rudy 1991 sheet in single
+precision (Rush-Larsen, dt = 0.01 ms, gx = gy = 9.5, S1 planar stimulus on
+the left 4 columns, V of every cell logged every 2 ms).
+
+One bench "step" is one ``run_block`` call covering ``--segment-ms`` of
+simulated time (default 20 ms = 2000 time steps of 1,048,576 cells). The
+state stays resident on the device between steps; the log is copied back to
+the host once per batch inside the timed region.
+
+Prints one JSON line with the throughput, the HBM roofline of the dominant
+kernel (algorithmic bytes per launch / mean launch time from HIP events),
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(nx, ny, precision):
+    import myokit_amd as myokit
+    m = myokit.load_model('example')
+    p = myokit.Protocol()
+    p.schedule(1, 0, 2)          # S1 at t = 0, 2 ms
+    p.schedule(1, 320, 2)        # S2 at t = 320 ms, 2 ms
+    s = myokit.SimulationOpenCL(m, p, ncells=(nx, ny), rl=True,
+                                precision=precision)
+    s.set_step_size(0.01)
+    s.set_conductance(9.5, 9.5)
+    s.set_paced_cells(4, ny, 0, 0)
+    return s
+
+
+def cpu_baseline(nx, ny, sample_ms, threads):
+    """The CPU oracle (double-precision C, OpenMP) on the first
+    ``sample_ms`` of the same workload."""
+    import myokit_amd as myokit
+    from oracle.cgen import OracleSimulation
+    m = myokit.load_model('example')
+    p = myokit.Protocol()
+    p.schedule(1, 0, 2)
+    o = OracleSimulation(m, p, ncells=(nx, ny), rl=True)
+    o.dt = 0.01
+    o.gx = o.gy = 9.5
+    o.set_paced_cells(4, ny, 0, 0)
+    o.threads = threads
+    # warm up the build
+    o.run(0.01, log_vars=['membrane.V'], log_interval=2.0, cells=[0])
+    t0 = time.perf_counter()
+    r = o.run(sample_ms, log_vars=['membrane.V'], log_interval=2.0)
+    el = time.perf_counter() - t0
+    return r['steps'] * nx * ny / el, r['steps'], el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--segment-ms', type=float, default=20.0)
+    ap.add_argument('--nx', type=int, default=1024)
+    ap.add_argument('--ny', type=int, default=1024)
+    ap.add_argument('--precision', type=int, default=32)
+    ap.add_argument('--cpu-sample-ms', type=float, default=0.5)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU'
+              ' path (see DESIGN.md for how the sheet would shard).',
+              file=sys.stderr)
+        sys.exit(2)
+
+    s = workload(args.nx, args.ny, args.precision)
+    seg = args.segment_ms
+    n = args.nx * args.ny
+    steps_per_seg = int(round(seg / 0.01))
+
+    for _ in range(args.warmup):
+        s.run_block(seg, ['membrane.V'], log_interval=2.0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = s.run_block(seg, ['membrane.V'], log_interval=2.0)
+    el = time.perf_counter() - t0
+    value = args.steps * steps_per_seg * n / el
+    V = out['membrane.V']
+    assert np.all(np.isfinite(V))
+
+    # Roofline leg: one more segment with every launch bracketed by events
+    s.set_engine_option('time_kernels', 1)
+    s.engine_timing(reset=True)
+    s.run_block(seg, ['membrane.V'], log_interval=2.0)
+    tm = s.engine_timing(reset=True)
+    s.set_engine_option('time_kernels', 0)
+    w = 4 if args.precision == 32 else 8
+    bytes_per_cell_step = 2 * 8 * w       # state in + out, S = 8
+    cells_per_launch = tm['cell_steps'] / max(tm['launches'], 1)
+    avg_ms = tm['kernel_ms'] / max(tm['launches'], 1)
+    achieved = bytes_per_cell_step * cells_per_launch / (avg_ms * 1e-3) / 1e9
+
+    line = {
+        'metric': 'cell-steps/s (2D 1024² LR1991 fp32 fwd); '
+                  'fwd+adjoint vs fwd time ratio',
+        'value': value,
+        'unit': 'cell-steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': el / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32' if args.precision == 32 else 'f64',
+        'data': 'synthetic',
+        'config': {
+            'workload': '2D sheet %dx%d LR1991 (example.mmt), Rush-Larsen, '
+                        'dt=0.01 ms, gx=gy=9.5, S1 2 ms on left 4 columns, '
+                        'V of all cells logged every 2 ms; one step = %g ms '
+                        '(%d time steps)' % (args.nx, args.ny, seg,
+                                              steps_per_seg),
+            'cells': n, 'time_steps_per_step': steps_per_seg,
+            'precision': args.precision,
+        },
+        'roofline': {
+            'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+            'traffic': None,
+            'kernel': 'mk_grid', 'avg_launch_ms': avg_ms,
+            'bytes_per_cell_step': bytes_per_cell_step,
+            'cells_per_launch': cells_per_launch,
+        },
+    }
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cv, csteps, cel = cpu_baseline(args.nx, args.ny, args.cpu_sample_ms,
+                                       threads)
+        line['cpu_baseline'] = {
+            'value': cv, 'unit': 'cell-steps/s', 'cores': threads,
+            'kind': 'port',
+            'sample': 'CPU oracle (double-precision C, OpenMP) on the first '
+                      '%g ms (%d time steps) of the same %dx%d workload, '
+                      '%.1f s' % (args.cpu_sample_ms, csteps, args.nx,
+                                  args.ny, cel),
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

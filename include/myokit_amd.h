@@ -1,0 +1,194 @@
+/*
+ * myokit_amd.h -- C ABI of the MI355X tissue-simulation library
+ * (libmyokit_amd.so).
+ *
+ * This is the boundary that replaces the per-run CPython extension which
+ * myokit's SimulationOpenCL compiles from myokit/_sim/openclsim.c:
+ *
+ *   reference (myokit/_sim/openclsim.c)           this library
+ *   ---------------------------------------      -----------------------------
+ *   clCreateProgramWithSource/clBuildProgram      mkh_program_build
+ *     (:815-839)
+ *   sim_init(21 Python args) (:309-1031)          mkh_sim_create + mkh_sim_set_*
+ *                                                 + mkh_sim_init + mkh_sim_set_log
+ *   sim_step() -> time / ArithmeticError          mkh_sim_step
+ *     (:1036-1211)
+ *   sim_clean() (:216-292, :1216-1221)            mkh_sim_clean
+ *   final state read into state_out (:1185-1190)  mkh_sim_get_state
+ *   mcl_select_device / mcl_info (mcl.h:265,823)  mkh_device_count/_info
+ *
+ * All entry points take plain pointers and sizes. They return 0 on success
+ * or a negative MKH_E_* code; mkh_last_error() then holds a message. A
+ * simulation handle owns its device memory. Handles are independent: several
+ * may exist per process and per device. A single handle is not thread-safe.
+ *
+ * Host arrays use the reference's layouts: state is cell-major
+ * (state[cell * n_state + i], cell = x + y * nx, openclsim.cl:316-319); field
+ * data is cell-major and field-minor (openclsim.py:1082-1089). On the device,
+ * state and fields are stored structure-of-arrays.
+ */
+#ifndef MYOKIT_AMD_H
+#define MYOKIT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes */
+#define MKH_OK 0
+#define MKH_E_ARG -1          /* invalid argument */
+#define MKH_E_HIP -2          /* HIP runtime error (message has the name) */
+#define MKH_E_COMPILE -3      /* hiprtc failed to compile the kernel */
+#define MKH_E_STATE -4        /* call out of order (e.g. step before init) */
+#define MKH_E_NAN -5          /* NaN in cell 0 at a log point: run halted */
+#define MKH_E_MEMORY -6       /* allocation failed */
+/* Pacing errors, same meaning as the ESys flags of myokit/_sim/pacing.h:47-64 */
+#define MKH_E_PACING_NEGATIVE_TIME -40
+#define MKH_E_PACING_SIMULTANEOUS -50
+#define MKH_E_PACING_INVALID -20
+
+/* Coupling modes (which diffusion variant the kernel computes) */
+#define MKH_COUPLING_NONE 0      /* diffusion disabled, or a single cell */
+#define MKH_COUPLING_STENCIL 1   /* homogeneous 1D/2D grid (diff_step, openclsim.cl:384) */
+#define MKH_COUPLING_HETERO 2    /* conductance fields (diff_hetero, :452) */
+#define MKH_COUPLING_GRAPH 3     /* set_connections (diff_arb_*, :499-573) */
+
+/* Execution drivers */
+#define MKH_DRIVER_CELLS 0  /* one thread per uncoupled cell, all steps per launch */
+#define MKH_DRIVER_BLOCK 1  /* one workgroup per domain, all steps per launch */
+#define MKH_DRIVER_GRID 2   /* whole grid, one fused step per launch */
+
+/* Simulation geometry and kernel layout. */
+typedef struct mkh_sim_config {
+    int precision;         /* 32 or 64 (myokit.SINGLE/DOUBLE_PRECISION) */
+    int n_state;           /* states per cell */
+    int n_field;           /* per-cell scalar fields */
+    int n_quant;           /* per-cell logged quantities compiled into the kernel */
+    int i_vm;              /* index of the membrane potential state */
+    int coupling;          /* MKH_COUPLING_* */
+    int driver;            /* MKH_DRIVER_* */
+    int block_threads;     /* threads per workgroup */
+    long long nx, ny;      /* grid size (1D: ny = 1) */
+    int device;            /* HIP device ordinal */
+    int reserved;
+} mkh_sim_config;
+
+/* Library */
+const char* mkh_version(void);
+const char* mkh_last_error(void);
+int mkh_device_count(int* n);
+int mkh_device_info(int device, char* name, int name_len,
+                    long long* total_mem, int* n_cu);
+
+/* Kernel program: compile generated HIP source with hiprtc for the device's
+ * architecture (gfx950) and load it. Replaces clCreateProgramWithSource +
+ * clBuildProgram (openclsim.c:815-839). If cache_dir is non-NULL, code
+ * objects are cached there by a hash of source and options. build_log (may
+ * be NULL) receives the compiler log. */
+int mkh_program_build(const char* source, const char* options,
+                      const char* cache_dir, int device, void** program,
+                      char* build_log, int build_log_len);
+int mkh_program_free(void* program);
+
+/* Simulation object (device buffers for one tissue). */
+int mkh_sim_create(void* program, const mkh_sim_config* cfg, void** sim);
+int mkh_sim_destroy(void* sim);
+
+/* State in and out (host AoS doubles, n = n_cells * n_state). Replaces the
+ * PyList conversions of openclsim.c:512-524 and :1185-1190. */
+int mkh_sim_set_state(void* sim, const double* state, long long n);
+int mkh_sim_get_state(void* sim, double* state, long long n);
+
+/* Per-cell fields, cell-major / field-minor (openclsim.py:1082-1089). */
+int mkh_sim_set_fields(void* sim, const double* data, long long n);
+
+/* Conductances (openclsim.py:1286-1449). Connections are (i, j, g) with
+ * i < j, validated by the caller; summation order per cell follows the
+ * order of the list. */
+int mkh_sim_set_conductance(void* sim, double gx, double gy);
+int mkh_sim_set_conductance_field(void* sim, const double* gx, long long ngx,
+                                  const double* gy, long long ngy);
+int mkh_sim_set_connections(void* sim, const long long* i,
+                            const long long* j, const double* g,
+                            long long n);
+
+/* Independent domains for MKH_DRIVER_BLOCK: domain d owns cells
+ * [offsets[d], offsets[d+1]); n_domains + 1 offsets. Each domain is a 1D
+ * chain (stencil/hetero) or a connected set (graph). Without this call the
+ * whole grid is one domain. */
+int mkh_sim_set_domains(void* sim, const long long* offsets,
+                        long long n_domains);
+
+/* Paced cells (calculate_pacing, openclsim.cl:249-280). */
+int mkh_sim_set_pacing_all(void* sim);
+int mkh_sim_set_pacing_none(void* sim);
+int mkh_sim_set_pacing_rect(void* sim, long long nx, long long ny,
+                            long long x, long long y);
+int mkh_sim_set_pacing_list(void* sim, const long long* cells, long long n);
+
+/* Logging: colmap[q * n_cells + cell] is the log column that per-cell
+ * quantity q of that cell is written to, or -1. n_cols is the number of
+ * per-cell columns. nan_check enables the reference's NaN test on state 0
+ * of cell 0 at every log point (openclsim.c:1079-1090). */
+int mkh_sim_set_logging(void* sim, const int* colmap, long long n_quant,
+                        int n_cols, int nan_check);
+
+/* Begin a run from tmin to tmax with step dt (sim_init, openclsim.c:309).
+ * events: n_events rows of (level, start, duration, period, multiplier),
+ * in protocol order. Returns the number of log rows the run will produce
+ * in *n_log_rows (the time-step rule is deterministic). */
+int mkh_sim_init(void* sim, double tmin, double tmax, double dt,
+                 const double* events, long long n_events,
+                 double log_interval, long long* n_log_rows);
+
+/* Host destinations for the log: values[row * n_cols + col] (precision of
+ * the simulation), times[row] and paces[row] as doubles holding the
+ * precision-cast engine time and pacing level. Any may be NULL if unused. */
+int mkh_sim_set_log(void* sim, void* values, double* times, double* paces);
+
+/* Advance by one bounded batch of steps (sim_step, openclsim.c:1036).
+ * *time receives the engine time; *status is 0 (more to do), 1 (finished)
+ * or 2 (halted on NaN, in which case MKH_E_NAN is also returned). */
+int mkh_sim_step(void* sim, double* time, int* status);
+
+/* Release the per-run state (sim_clean, openclsim.c:216). Device state
+ * stays resident. */
+int mkh_sim_clean(void* sim);
+
+/* Tuning and measurement options:
+ *   "batch_steps"   max steps per mkh_sim_step call
+ *   "log_rows"      device log buffer rows per batch
+ *   "time_kernels"  1: bracket every launch with HIP events
+ *   "block_threads" threads per workgroup for the next runs */
+int mkh_sim_set_option(void* sim, const char* key, long long value);
+
+/* Kernel timing since the last reset (time_kernels = 1): total milliseconds
+ * inside kernels, number of launches and cell-steps computed. */
+int mkh_sim_get_timing(void* sim, double* kernel_ms, long long* launches,
+                       long long* cell_steps);
+int mkh_sim_reset_timing(void* sim);
+
+/* Number of log rows produced so far in the current run. */
+int mkh_sim_log_rows(void* sim, long long* rows);
+
+/* Host-only checks of the pacing system and the time-step planner (no
+ * device needed). Counterpart of the reference's test harness
+ * myokit/tests/ansic_event_based_pacing.c, which exposes ESys to Python.
+ *   mkh_pacing_levels: creates the event system at t0, then for each
+ *     times[k] (non-decreasing) advances and stores the level and the next
+ *     event time.
+ *   mkh_plan: the step sequence of a run (engine time, step size, pacing
+ *     level, log flag per step), at most max_steps steps. */
+int mkh_pacing_levels(const double* events, long long n_events, double t0,
+                      const double* times, long long n_times, double* levels,
+                      double* next_times);
+int mkh_plan(double tmin, double tmax, double dt, const double* events,
+             long long n_events, double log_interval, double* t,
+             double* step, double* pace, int* log, long long max_steps,
+             long long* n_steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MYOKIT_AMD_H */

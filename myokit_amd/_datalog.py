@@ -1,0 +1,347 @@
+"""
+Simulation logs.
+
+``DataLog`` is a dict of ``key -> column`` like ``myokit.DataLog``
+(``myokit/_datalog.py:42``) with the same key grammar: ``"engine.time"`` for
+global variables, ``"3.membrane.V"`` (1D) or ``"3.1.membrane.V"`` (2D, x
+first) for per-cell ones. Columns produced by the engine are numpy arrays
+(float32 in single precision, float64 in double, as the reference's
+``array('f')`` / ``array('d')``), which avoids the reference's per-value Python
+appends at large cell counts.
+
+``prepare_log`` follows ``myokit.prepare_log`` (``_datalog.py:1783-2193``):
+flags, lists of names, and existing logs, with the same checks.
+"""
+import re
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _expressions as ex
+
+LOG_NONE = 0
+LOG_STATE = 1
+LOG_BOUND = 2
+LOG_INTER = 4
+LOG_DERIV = 8
+LOG_ALL = LOG_STATE + LOG_INTER + LOG_BOUND + LOG_DERIV
+
+SINGLE_PRECISION = 32
+DOUBLE_PRECISION = 64
+
+_ID_NAME = re.compile(r'(\d+\.)+')
+
+
+def split_key(key):
+    """Splits ``"1.2.membrane.V"`` into ``("1.2.", "membrane.V")``."""
+    m = _ID_NAME.match(key, 0)
+    if m:
+        return key[:m.end()], key[m.end():]
+    return '', key
+
+
+def dimco(*dims):
+    """Yields all index tuples of a grid, first dimension changing fastest:
+    ``dimco(2, 3)`` gives (0,0), (1,0), (0,1), (1,1), (0,2), (1,2)."""
+    if len(dims) == 0:
+        return
+    grids = np.indices(tuple(reversed(dims))).reshape(len(dims), -1)
+    for idx in grids.T:
+        yield tuple(int(i) for i in reversed(idx))
+
+
+class DataLog(OrderedDict):
+    """An ordered dict of logged columns, with an optional time key."""
+
+    def __init__(self, other=None, time=None):
+        super().__init__()
+        self._time = None
+        self.cmeta = _MetaDict()
+        if other is not None:
+            for k, v in other.items():
+                self[k] = v
+            if isinstance(other, DataLog):
+                self._time = other._time
+        if time is not None:
+            self.set_time_key(time)
+
+    def time_key(self):
+        return self._time
+
+    def set_time_key(self, key):
+        if key is not None and key not in self:
+            raise ValueError('Time key not found in log: ' + str(key))
+        self._time = key
+
+    def time(self):
+        if self._time is None:
+            raise ValueError('No time variable set in log.')
+        return self[self._time]
+
+    def length(self):
+        for v in self.values():
+            return len(v)
+        return 0
+
+    def __len__(self):
+        return OrderedDict.__len__(self)
+
+    def has_nan(self):
+        """True if any column contains a NaN or infinity as its last value
+        or anywhere."""
+        for v in self.values():
+            a = np.asarray(v, dtype=float)
+            if a.size and not np.all(np.isfinite(a)):
+                return True
+        return False
+
+    def npview(self):
+        """Returns a DataLog with every column as a numpy array."""
+        d = DataLog()
+        for k, v in self.items():
+            d[k] = np.asarray(v)
+        d._time = self._time
+        return d
+
+    def validate(self):
+        n = None
+        for k, v in self.items():
+            if n is None:
+                n = len(v)
+            elif len(v) != n:
+                raise ValueError(
+                    'All columns in a DataLog must have the same length.')
+
+    def variable_info(self):
+        """Maps each variable name to the sorted list of cell indices it is
+        logged for (``()`` for global variables)."""
+        info = OrderedDict()
+        for k in self.keys():
+            idx, name = split_key(k)
+            t = tuple(int(x) for x in idx.split('.')[:-1]) if idx else ()
+            info.setdefault(name, []).append(t)
+        return info
+
+    def extend_column(self, key, values):
+        """Appends ``values`` to a column, whatever the column's type."""
+        cur = self[key]
+        if isinstance(cur, np.ndarray):
+            OrderedDict.__setitem__(
+                self, key, np.concatenate([cur, np.asarray(
+                    values, dtype=cur.dtype)]))
+        elif hasattr(cur, 'extend'):
+            cur.extend(values)
+        else:
+            for x in values:
+                cur.append(x)
+
+    def trim(self, n):
+        """Keeps the first ``n`` entries of every column."""
+        for k in list(self.keys()):
+            v = self[k]
+            if isinstance(v, np.ndarray):
+                OrderedDict.__setitem__(self, k, v[:n])
+            else:
+                del v[n:]
+
+
+class _MetaDict(dict):
+    def __missing__(self, key):
+        d = self[key] = {}
+        return d
+
+
+def prepare_log(log, model, dims=None, global_vars=None,
+                if_empty=LOG_NONE, allowed_classes=LOG_ALL,
+                precision=DOUBLE_PRECISION):
+    """Returns a DataLog for a simulation, built from a flag, a sequence of
+    names, or an existing log (checked for compatibility)."""
+    dtype = np.float64 if precision == DOUBLE_PRECISION else np.float32
+    dims = tuple(dims) if dims else ()
+    if dims:
+        dcombos = ['.'.join(str(i) for i in x) + '.' for x in dimco(*dims)]
+    else:
+        dcombos = ['']
+    global_vars = list(global_vars or [])
+    for name in global_vars:
+        try:
+            v = model.get(name)
+        except KeyError:
+            raise ValueError(
+                'Unknown variable specified in global_vars <' + name + '>.')
+        if v.is_state():
+            raise ValueError('State cannot be global variable.')
+
+    def check_class(var):
+        if var.is_constant():
+            raise ValueError(
+                f'This log does not support constants, got <{var}>.')
+        elif var.is_state():
+            if not LOG_STATE & allowed_classes:
+                raise ValueError(
+                    f'This log does not support state variables, got <{var}>.')
+        elif var.is_bound():
+            if not LOG_BOUND & allowed_classes:
+                raise ValueError(
+                    f'This log does not support bound variables, got <{var}>.')
+        elif not LOG_INTER & allowed_classes:
+            raise ValueError('This log does not support intermediary'
+                             f' variables, got <{var}>.')
+
+    def new():
+        return np.zeros(0, dtype=dtype)
+
+    if log is None:
+        if if_empty & allowed_classes == if_empty:
+            log = if_empty
+        else:
+            raise ValueError(
+                'The option given for `if_empty` should be an allowed class.')
+
+    if type(log) == int:
+        flag = log
+        log = DataLog()
+        if flag == LOG_ALL:
+            flag = allowed_classes
+        if LOG_STATE & flag:
+            if not LOG_STATE & allowed_classes:
+                raise ValueError('This log does not support state variables.')
+            for s in model.states():
+                for c in dcombos:
+                    log[c + s.qname()] = new()
+            flag -= LOG_STATE
+        if LOG_BOUND & flag:
+            if not LOG_BOUND & allowed_classes:
+                raise ValueError('This log does not support bound variables.')
+            for label, var in model.bindings():
+                name = var.qname()
+                if name in global_vars:
+                    log[name] = new()
+                else:
+                    for c in dcombos:
+                        log[c + name] = new()
+            flag -= LOG_BOUND
+        if LOG_INTER & flag:
+            if not LOG_INTER & allowed_classes:
+                raise ValueError(
+                    'This log does not support intermediary variables.')
+            for var in model.variables(inter=True, deep=True):
+                name = var.qname()
+                if name in global_vars:
+                    log[name] = new()
+                else:
+                    for c in dcombos:
+                        log[c + name] = new()
+            flag -= LOG_INTER
+        if LOG_DERIV & flag:
+            if not LOG_DERIV & allowed_classes:
+                raise ValueError('This log does not support time-derivatives.')
+            for var in model.states():
+                for c in dcombos:
+                    log['dot(' + c + var.qname() + ')'] = new()
+            flag -= LOG_DERIV
+        if flag != 0:
+            raise ValueError('One or more unknown flags given as log.')
+        t = model.time()
+        if t is not None and t.qname() in log:
+            log.set_time_key(t.qname())
+        return log
+
+    if isinstance(log, dict):
+        if not isinstance(log, DataLog):
+            log = DataLog(log)
+        t = model.time()
+        if t is not None and t.qname() in log:
+            log.set_time_key(t.qname())
+        log.validate()
+        for key in list(log.keys()):
+            deriv = key[0:4] == 'dot(' and key[-1:] == ')'
+            k = key[4:-1] if deriv else key
+            kdims, kname = split_key(k)
+            try:
+                var = model.get(kname)
+            except KeyError:
+                raise ValueError(f'Unknown variable <{kname}> in log.')
+            if deriv:
+                if not LOG_DERIV & allowed_classes:
+                    raise ValueError(
+                        f'This log does not support derivatives, got <{key}>.')
+                if not var.is_state():
+                    raise ValueError('Cannot log time derivative of non-state'
+                                     f' <{var.qname()}>.')
+            else:
+                check_class(var)
+            if kdims:
+                if kname in global_vars:
+                    raise ValueError('Cannot specify a cell index for global'
+                                     f' logging variable <{kname}>.')
+                if kdims not in dcombos:
+                    raise ValueError(f'Invalid index <{kdims}> in log.')
+            elif dims:
+                if kname not in global_vars:
+                    raise ValueError('DataLog contains non-indexed entry for'
+                                     f' cell-specific variable <{kname}>.')
+        for v in log.values():
+            if not (isinstance(v, np.ndarray) or (
+                    hasattr(v, 'append') and callable(v.append))):
+                raise ValueError('Logging dict must map qnames to objects'
+                                 ' that support the append() method.')
+        return log
+
+    try:
+        if len(log) > 0:
+            log[0]
+    except Exception:
+        raise ValueError('Argument `log` has unexpected type. Expecting None,'
+                         ' integer flag, sequence of names, dict or DataLog.')
+    if isinstance(log, str):
+        raise ValueError('String passed in as `log` argument, should be list'
+                         ' or other sequence containing strings.')
+    lst = log
+    log = DataLog()
+    checked = {}
+    from ._model import Variable
+    for key in lst:
+        if isinstance(key, Variable):
+            key = key.qname()
+        elif isinstance(key, ex.LhsExpression):
+            key = str(key)
+        deriv = key[0:4] == 'dot(' and key[-1:] == ')'
+        if deriv:
+            key = key[4:-1]
+        kdims, kname = split_key(key)
+        try:
+            var = checked[kname]
+        except KeyError:
+            try:
+                var = model.get(kname)
+            except KeyError:
+                raise ValueError(f'Unknown variable <{kname}> in list.')
+            if deriv:
+                if not LOG_DERIV & allowed_classes:
+                    raise ValueError(
+                        f'This log does not support derivatives, got <{key}>.')
+                if not var.is_state():
+                    raise ValueError('Cannot log time derivative of non-state'
+                                     f' <{var.qname()}>.')
+            else:
+                check_class(var)
+            checked[kname] = var
+        if kdims:
+            if kname in global_vars:
+                raise ValueError('Cannot specify a cell index for global'
+                                 f' logging variable <{kname}>.')
+            if kdims not in dcombos:
+                raise ValueError(f'Invalid index <{kdims}> in list.')
+            k = kdims + kname
+            log['dot(' + k + ')' if deriv else k] = new()
+        elif kname in global_vars:
+            log['dot(' + kname + ')' if deriv else kname] = new()
+        else:
+            for c in dcombos:
+                k = c + kname
+                log['dot(' + k + ')' if deriv else k] = new()
+    t = model.time()
+    if t is not None and t.qname() in log:
+        log.set_time_key(t.qname())
+    return log

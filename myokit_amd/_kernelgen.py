@@ -1,0 +1,426 @@
+"""
+HIP code generation for the tissue engine.
+
+``HipExpressionWriter`` turns model expressions into HIP C++ (the role of
+``myokit.formats.opencl.OpenCLExpressionWriter``, ``formats/opencl/
+_ewriter.py:12-111``, on top of the C semantics of ``formats/ansic/
+_ewriter.py:12-121``: ``pow``, ``floor``-based quotient and remainder,
+ternary ``if``/``piecewise``, ``log(a)/log(b)``). ``generate_kernel`` writes
+one device function ``mk_cell`` that evaluates the model and advances one
+cell by one step (the body of ``cell_step``, ``_sim/openclsim.cl:298-367``),
+then appends the fixed driver kernels of ``csrc/mk_drivers.hip``.
+
+Differences from the reference's kernel text, by design:
+
+* constants (literal and calculated, unless a field depends on them) are
+  evaluated once on the host in double precision and written as exact
+  hexadecimal literals (the reference writes ``#define`` expressions that the
+  device evaluates, ``openclsim.cl:173-186``);
+* double precision uses the portable functions of ``csrc/mk_math.h`` so the
+  device matches the CPU oracle bit for bit; single precision uses the
+  device's float functions (``__expf`` etc. when ``native_maths`` is set, the
+  analogue of OpenCL's ``native_*``);
+* integer powers with a literal exponent ``|n| <= 64`` use binary powering
+  (``mk_powi``) instead of a general ``pow``;
+* state is read from and written to SoA registers, not AoS memory.
+"""
+import math
+import os
+
+import numpy as np
+
+from . import _expressions as ex
+
+_CSRC = os.path.join(os.path.dirname(__file__), 'csrc')
+
+SINGLE_PRECISION = 32
+DOUBLE_PRECISION = 64
+
+
+def _read(name):
+    with open(os.path.join(_CSRC, name), 'r') as f:
+        return f.read()
+
+
+def real_literal(value, precision):
+    """Exact C literal for ``value`` in the given precision."""
+    value = float(value)
+    if precision == SINGLE_PRECISION:
+        value = float(np.float32(value))
+    if math.isnan(value):
+        return '(Real)__builtin_nan("")'
+    if math.isinf(value):
+        return '(Real)__builtin_inf()' if value > 0 \
+            else '(-(Real)__builtin_inf())'
+    s = value.hex()
+    if precision == SINGLE_PRECISION:
+        s += 'f'
+    if value < 0 or s.startswith('-'):
+        return '(' + s + ')'
+    return s
+
+
+# Function names per precision: (double, float, float-native)
+_FUNCS = {
+    'sqrt': ('__builtin_sqrt', 'sqrtf', 'sqrtf'),
+    'exp': ('mk_exp', 'expf', '__expf'),
+    'log': ('mk_log', 'logf', '__logf'),
+    'log10': ('mk_log10', 'log10f', '__log10f'),
+    'sin': ('mk_sin', 'sinf', '__sinf'),
+    'cos': ('mk_cos', 'cosf', '__cosf'),
+    'tan': ('mk_tan', 'tanf', '__tanf'),
+    'asin': ('mk_asin', 'asinf', 'asinf'),
+    'acos': ('mk_acos', 'acosf', 'acosf'),
+    'atan': ('mk_atan', 'atanf', 'atanf'),
+    'floor': ('__builtin_floor', 'floorf', 'floorf'),
+    'ceil': ('__builtin_ceil', 'ceilf', 'ceilf'),
+    'abs': ('__builtin_fabs', 'fabsf', 'fabsf'),
+    'pow': ('mk_pow', 'powf', '__powf'),
+    'powi': ('mk_powi', 'mk_powif', 'mk_powif'),
+}
+
+
+class HipExpressionWriter:
+    """Writes expressions as HIP C++ source text.
+
+    ``names`` maps LhsExpressions to the C text that stands for them; a
+    constant without an entry is inlined as its (host-evaluated) value.
+    """
+
+    def __init__(self, precision=SINGLE_PRECISION, native_maths=False):
+        self.precision = precision
+        self.native = bool(native_maths)
+        self.names = {}
+        self.constants = {}
+
+    def fn(self, name):
+        d, f, fn = _FUNCS[name]
+        if self.precision == DOUBLE_PRECISION:
+            return d
+        return fn if self.native else f
+
+    def ex(self, e):
+        m = getattr(self, '_ex_' + type(e).__name__)
+        return m(e)
+
+    # Atoms
+    def _ex_Number(self, e):
+        return real_literal(e.eval(), self.precision)
+
+    def _ex_Name(self, e):
+        try:
+            return self.names[e]
+        except KeyError:
+            pass
+        var = e.var()
+        if var in self.constants:
+            return real_literal(self.constants[var], self.precision)
+        raise KeyError('No C name for <' + e.code() + '>')
+
+    def _ex_Derivative(self, e):
+        return self.names[e]
+
+    # Operators
+    def _ex_PrefixPlus(self, e):
+        return '(+' + self.ex(e[0]) + ')'
+
+    def _ex_PrefixMinus(self, e):
+        return '(-' + self.ex(e[0]) + ')'
+
+    def _infix(self, e, op):
+        return '(' + self.ex(e[0]) + ' ' + op + ' ' + self.ex(e[1]) + ')'
+
+    def _ex_Plus(self, e):
+        return self._infix(e, '+')
+
+    def _ex_Minus(self, e):
+        return self._infix(e, '-')
+
+    def _ex_Multiply(self, e):
+        return self._infix(e, '*')
+
+    def _ex_Divide(self, e):
+        return self._infix(e, '/')
+
+    def _ex_Quotient(self, e):
+        # Rounds towards minus infinity, as myokit.Quotient
+        return self.fn('floor') + '(' + self._infix(e, '/') + ')'
+
+    def _ex_Remainder(self, e):
+        a, b = self.ex(e[0]), self.ex(e[1])
+        return ('(' + a + ' - ' + b + ' * ' + self.fn('floor') + '(' + a
+                + ' / ' + b + '))')
+
+    def _ex_Power(self, e):
+        a, b = e
+        if isinstance(b, ex.Number):
+            v = b.eval()
+            if v == int(v) and abs(v) <= 64:
+                return (self.fn('powi') + '(' + self.ex(a) + ', '
+                        + str(int(v)) + ')')
+        return self.fn('pow') + '(' + self.ex(a) + ', ' + self.ex(b) + ')'
+
+    # Functions
+    def _func(self, name, e):
+        return self.fn(name) + '(' + ', '.join(self.ex(x) for x in e) + ')'
+
+    def _ex_Sqrt(self, e):
+        return self._func('sqrt', e)
+
+    def _ex_Exp(self, e):
+        return self._func('exp', e)
+
+    def _ex_Log(self, e):
+        if len(e) == 1:
+            return self._func('log', e)
+        f = self.fn('log')
+        return ('(' + f + '(' + self.ex(e[0]) + ') / ' + f + '('
+                + self.ex(e[1]) + '))')
+
+    def _ex_Log10(self, e):
+        return self._func('log10', e)
+
+    def _ex_Sin(self, e):
+        return self._func('sin', e)
+
+    def _ex_Cos(self, e):
+        return self._func('cos', e)
+
+    def _ex_Tan(self, e):
+        return self._func('tan', e)
+
+    def _ex_ASin(self, e):
+        return self._func('asin', e)
+
+    def _ex_ACos(self, e):
+        return self._func('acos', e)
+
+    def _ex_ATan(self, e):
+        return self._func('atan', e)
+
+    def _ex_Floor(self, e):
+        return self._func('floor', e)
+
+    def _ex_Ceil(self, e):
+        return self._func('ceil', e)
+
+    def _ex_Abs(self, e):
+        return self._func('abs', e)
+
+    def _ex_If(self, e):
+        return ('(' + self.ex(e[0]) + ' ? ' + self.ex(e[1]) + ' : '
+                + self.ex(e[2]) + ')')
+
+    def _ex_Piecewise(self, e):
+        ops = list(e)
+        out = []
+        for i in range(0, len(ops) - 1, 2):
+            out.append('(' + self.ex(ops[i]) + ' ? ' + self.ex(ops[i + 1])
+                       + ' : ')
+        out.append(self.ex(ops[-1]))
+        out.append(')' * ((len(ops) - 1) // 2))
+        return ''.join(out)
+
+    # Conditions
+    def _ex_Not(self, e):
+        return '(!' + self.ex(e[0]) + ')'
+
+    def _ex_Equal(self, e):
+        return self._infix(e, '==')
+
+    def _ex_NotEqual(self, e):
+        return self._infix(e, '!=')
+
+    def _ex_More(self, e):
+        return self._infix(e, '>')
+
+    def _ex_Less(self, e):
+        return self._infix(e, '<')
+
+    def _ex_MoreEqual(self, e):
+        return self._infix(e, '>=')
+
+    def _ex_LessEqual(self, e):
+        return self._infix(e, '<=')
+
+    def _ex_And(self, e):
+        return self._infix(e, '&&')
+
+    def _ex_Or(self, e):
+        return self._infix(e, '||')
+
+
+class CellModel:
+    """The per-cell computation extracted from a prepared model.
+
+    ``model``      cloned model (RL-converted if needed, bindings prepared,
+                   unique names created)
+    ``bound``      {variable: 'time' | 'pace' | 'idiff'}
+    ``rl_states``  {state: (inf_variable, tau_variable)}
+    ``fields``     ordered variables replaced by per-cell values
+    ``quantities`` ordered per-cell logged variables (states, intermediaries,
+                   or the variable bound to the diffusion current)
+    """
+
+    def __init__(self, model, bound, rl_states, fields, quantities):
+        self.model = model
+        self.bound = dict(bound)
+        self.rl_states = dict(rl_states)
+        self.fields = list(fields)
+        self.quantities = list(quantities)
+        self.states = list(model.states())
+        for q in self.quantities:
+            if not (q.is_state() or q.is_intermediary() or q in self.bound):
+                raise ValueError('Cannot log <' + q.qname() + '>.')
+
+        # Constants: host-evaluated unless they depend on a field
+        fielded = set(self.fields)
+        self.per_cell_constants = []
+        self.constants = {}
+        subst = {}
+        for var in model.constants_order():
+            if var in fielded:
+                continue
+            deps = [r.var() for r in var.rhs().references()]
+            if any(d in fielded for d in deps) or any(
+                    d in set(self.per_cell_constants) for d in deps):
+                self.per_cell_constants.append(var)
+                continue
+            value = var.rhs()._eval(subst)
+            subst[ex.Name(var)] = value
+            self.constants[var] = value
+        self.order = model.solvable_order()
+
+    def literal_constants(self):
+        return dict(self.constants)
+
+
+def _cname(var, prefix='v_'):
+    return prefix + var.uname()
+
+
+def generate_cell_function(cm, precision, native_maths=False):
+    """Returns the HIP source of ``mk_cell<LOG>`` for a CellModel."""
+    w = HipExpressionWriter(precision, native_maths)
+    w.constants = cm.constants
+    lines = []
+    tab = '    '
+    bound_names = {'time': 't', 'pace': 'pace', 'idiff': 'idiff'}
+    for var, what in cm.bound.items():
+        w.names[ex.Name(var)] = bound_names[what]
+    for i, s in enumerate(cm.states):
+        w.names[ex.Name(s)] = _cname(s, 'x_')
+        w.names[ex.Derivative(ex.Name(s))] = _cname(s, 'd_')
+    for k, var in enumerate(cm.fields):
+        w.names[ex.Name(var)] = 'fv[' + str(k) + ']'
+    for var in cm.per_cell_constants:
+        w.names[ex.Name(var)] = _cname(var, 'c_')
+    for var in cm.order:
+        if not var.is_state():
+            w.names[ex.Name(var)] = _cname(var)
+
+    lines.append('template <bool LOG>')
+    lines.append('__device__ __forceinline__ void mk_cell(')
+    lines.append(tab + 'Real* __restrict__ y, const Real t, const Real dt,'
+                 ' const Real pace,')
+    lines.append(tab + 'const Real idiff, const Real* __restrict__ fv,'
+                 ' Real* __restrict__ q)')
+    lines.append('{')
+    lines.append(tab + '(void)t; (void)dt; (void)pace; (void)idiff;'
+                 ' (void)fv; (void)q;')
+    for i, s in enumerate(cm.states):
+        lines.append(tab + 'const Real ' + _cname(s, 'x_') + ' = y['
+                     + str(i) + '];')
+    for var in cm.per_cell_constants:
+        lines.append(tab + 'const Real ' + _cname(var, 'c_') + ' = '
+                     + w.ex(var.rhs()) + ';')
+    rl = cm.rl_states
+    for var in cm.order:
+        if var.is_state():
+            if var in rl:
+                continue
+            lines.append(tab + 'const Real ' + _cname(var, 'd_') + ' = '
+                         + w.ex(var.rhs()) + ';')
+        else:
+            lines.append(tab + 'const Real ' + _cname(var) + ' = '
+                         + w.ex(var.rhs()) + ';')
+    if cm.quantities:
+        lines.append(tab + 'if (LOG) {')
+        for k, var in enumerate(cm.quantities):
+            lines.append(2 * tab + 'q[' + str(k) + '] = '
+                         + w.ex(ex.Name(var)) + ';')
+        lines.append(tab + '}')
+    expf = w.fn('exp')
+    for i, s in enumerate(cm.states):
+        x = _cname(s, 'x_')
+        if s in rl:
+            inf, tau = rl[s]
+            inf, tau = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
+            # RL update x = inf - (inf - x) * exp(-dt / tau), as written by
+            # the reference (openclsim.cl:358-364)
+            lines.append(tab + 'y[' + str(i) + '] = ' + inf + ' - (' + inf
+                         + ' - ' + x + ') * ' + expf + '(-dt / ' + tau + ');')
+        else:
+            lines.append(tab + 'y[' + str(i) + '] = ' + x + ' + dt * '
+                         + _cname(s, 'd_') + ';')
+    lines.append('}')
+    return '\n'.join(lines) + '\n'
+
+
+_FLOAT_HELPERS = '''
+__device__ __forceinline__ float mk_powif(float x, int n)
+{
+    float r = 1.0f, b = x;
+    const bool neg = n < 0;
+    unsigned int u = (unsigned int)(neg ? -n : n);
+    while (u) {
+        if (u & 1u) r = r * b;
+        u >>= 1;
+        if (u) b = b * b;
+    }
+    return neg ? 1.0f / r : r;
+}
+'''
+
+
+def generate_kernel(cm, precision, coupling, block_threads=256,
+                    native_maths=False):
+    """Returns the complete HIP source for hiprtc."""
+    if block_threads % 64 or not 64 <= block_threads <= 1024:
+        raise ValueError('block_threads must be a multiple of 64 in'
+                         ' [64, 1024].')
+    vm = cm.model.label('membrane_potential')
+    i_vm = vm.index() if (vm is not None and vm.is_state()) else 0
+    bcells = 2048 if precision == DOUBLE_PRECISION else 4096
+    real = 'double' if precision == DOUBLE_PRECISION else 'float'
+    parts = [
+        '// Generated by myokit_amd for model: '
+        + str(cm.model.name()).replace('\n', ' '),
+        '#define MK_NS ' + str(len(cm.states)),
+        '#define MK_NF ' + str(len(cm.fields)),
+        '#define MK_NQ ' + str(len(cm.quantities)),
+        '#define MK_IVM ' + str(i_vm),
+        '#define MK_COUPLING ' + str(int(coupling)),
+        '#define MK_BT ' + str(int(block_threads)),
+        '#define MK_BCELLS ' + str(bcells),
+        'typedef ' + real + ' Real;',
+        _read('mk_math.h'),
+        _read('mk_args.h'),
+        _FLOAT_HELPERS,
+        generate_cell_function(cm, precision, native_maths),
+        _read('mk_drivers.hip'),
+    ]
+    return '\n'.join(parts)
+
+
+def compile_options(precision):
+    """hiprtc options. Contraction is disabled in double precision so that
+    results match the CPU oracle bit for bit; single precision allows fused
+    multiply-adds. Packed FP32 instructions are disabled."""
+    opts = ['-O3', '-std=c++17',
+            '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+    if precision == DOUBLE_PRECISION:
+        opts.append('-ffp-contract=off')
+    else:
+        opts.append('-ffp-contract=fast')
+    return ' '.join(opts)

@@ -1,0 +1,135 @@
+"""
+ctypes binding to ``libmyokit_amd.so`` (C ABI: ``include/myokit_amd.h``).
+
+The library is built in-tree by ``myokit_amd/_build.py``. There is no
+fallback: if the library is missing, or no HIP device is present when a
+simulation is created, an error is raised.
+"""
+import ctypes
+import os
+
+from ._err import DeviceError
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        'libmyokit_amd.so')
+
+MKH_E_NAN = -5
+MKH_E_PACING_SIMULTANEOUS = -50
+MKH_E_PACING_NEGATIVE_TIME = -40
+
+COUPLING_NONE = 0
+COUPLING_STENCIL = 1
+COUPLING_HETERO = 2
+COUPLING_GRAPH = 3
+
+DRIVER_CELLS = 0
+DRIVER_BLOCK = 1
+DRIVER_GRID = 2
+
+# Every entry point declared in include/myokit_amd.h
+_c = ctypes
+_vp = _c.c_void_p
+_ll = _c.c_longlong
+_d = _c.c_double
+_i = _c.c_int
+_pd = _c.POINTER(_c.c_double)
+_pll = _c.POINTER(_c.c_longlong)
+_pi = _c.POINTER(_c.c_int)
+
+
+class SimConfig(ctypes.Structure):
+    _fields_ = [
+        ('precision', _i), ('n_state', _i), ('n_field', _i),
+        ('n_quant', _i), ('i_vm', _i), ('coupling', _i), ('driver', _i),
+        ('block_threads', _i), ('nx', _ll), ('ny', _ll), ('device', _i),
+        ('reserved', _i),
+    ]
+
+
+SIGNATURES = {
+    'mkh_version': (_c.c_char_p, []),
+    'mkh_last_error': (_c.c_char_p, []),
+    'mkh_device_count': (_i, [_pi]),
+    'mkh_device_info': (_i, [_i, _c.c_char_p, _i, _pll, _pi]),
+    'mkh_program_build': (_i, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _i,
+                               _c.POINTER(_vp), _c.c_char_p, _i]),
+    'mkh_program_free': (_i, [_vp]),
+    'mkh_sim_create': (_i, [_vp, _c.POINTER(SimConfig), _c.POINTER(_vp)]),
+    'mkh_sim_destroy': (_i, [_vp]),
+    'mkh_sim_set_state': (_i, [_vp, _pd, _ll]),
+    'mkh_sim_get_state': (_i, [_vp, _pd, _ll]),
+    'mkh_sim_set_fields': (_i, [_vp, _pd, _ll]),
+    'mkh_sim_set_conductance': (_i, [_vp, _d, _d]),
+    'mkh_sim_set_conductance_field': (_i, [_vp, _pd, _ll, _pd, _ll]),
+    'mkh_sim_set_connections': (_i, [_vp, _pll, _pll, _pd, _ll]),
+    'mkh_sim_set_domains': (_i, [_vp, _pll, _ll]),
+    'mkh_sim_set_pacing_all': (_i, [_vp]),
+    'mkh_sim_set_pacing_none': (_i, [_vp]),
+    'mkh_sim_set_pacing_rect': (_i, [_vp, _ll, _ll, _ll, _ll]),
+    'mkh_sim_set_pacing_list': (_i, [_vp, _pll, _ll]),
+    'mkh_sim_set_logging': (_i, [_vp, _pi, _ll, _i, _i]),
+    'mkh_sim_init': (_i, [_vp, _d, _d, _d, _pd, _ll, _d, _pll]),
+    'mkh_sim_set_log': (_i, [_vp, _vp, _pd, _pd]),
+    'mkh_sim_step': (_i, [_vp, _pd, _pi]),
+    'mkh_sim_clean': (_i, [_vp]),
+    'mkh_sim_set_option': (_i, [_vp, _c.c_char_p, _ll]),
+    'mkh_sim_get_timing': (_i, [_vp, _pd, _pll, _pll]),
+    'mkh_sim_reset_timing': (_i, [_vp]),
+    'mkh_sim_log_rows': (_i, [_vp, _pll]),
+    'mkh_pacing_levels': (_i, [_pd, _ll, _d, _pd, _ll, _pd, _pd]),
+    'mkh_plan': (_i, [_d, _d, _d, _pd, _ll, _d, _pd, _pd, _pd, _pi, _ll,
+                      _pll]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads the library (once) and declares every signature."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise ImportError(
+                'libmyokit_amd.so not found at ' + LIB_PATH + '. Build it'
+                ' with `python -m myokit_amd._build` (or'
+                ' __graft_entry__.build()).')
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().mkh_last_error().decode('utf-8', 'replace')
+
+
+def check(code, what=''):
+    """Raises DeviceError for a non-zero return code."""
+    if code != 0:
+        raise DeviceError((what + ': ' if what else '') + last_error()
+                          + ' (code ' + str(code) + ')')
+    return code
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().mkh_device_count(ctypes.byref(n))
+    return n.value
+
+
+def device_info(device=0):
+    name = ctypes.create_string_buffer(256)
+    mem = ctypes.c_longlong(0)
+    cu = ctypes.c_int(0)
+    check(lib().mkh_device_info(device, name, 256, ctypes.byref(mem),
+                                ctypes.byref(cu)), 'mkh_device_info')
+    return {'name': name.value.decode(), 'memory': mem.value,
+            'compute_units': cu.value}
+
+
+def ptr(array, ctype):
+    """Pointer to a contiguous numpy array's data."""
+    return array.ctypes.data_as(ctypes.POINTER(ctype))

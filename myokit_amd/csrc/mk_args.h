@@ -1,0 +1,50 @@
+/*
+ * mk_args.h -- argument block shared by the host library and the generated
+ * kernels (included by both; identical layout on host and device).
+ *
+ * Device layout: structure-of-arrays. State variable i of cell c lives at
+ * state[i * npad + c], field k at fields[k * npad + c]. Cells are numbered
+ * c = x + y * nx, as in the reference (openclsim.cl:316).
+ */
+#ifndef MK_ARGS_H
+#define MK_ARGS_H
+
+template <typename R>
+struct MkStepT {
+    R t;        /* engine time, cast to the simulation precision */
+    R dt;       /* step size, cast to the simulation precision */
+    R pace;     /* pacing level */
+    int row;    /* device log row written at this step, or -1 */
+};
+
+template <typename R>
+struct MkArgsT {
+    R* state_a;                 /* SoA state, buffer A */
+    R* state_b;                 /* SoA state, buffer B (grid driver) */
+    const R* fields;            /* SoA per-cell fields */
+    const MkStepT<R>* steps;    /* step table of the current batch */
+    R* logbuf;                  /* [rows][ncols] per-cell log values */
+    const int* colmap;          /* [nquant][npad] -> column or -1 */
+    const unsigned char* pace_mask;  /* [npad] 1 = paced (pace_mode 2) */
+    const int* csr_ptr;         /* graph: [n + 1] row pointers */
+    const int* csr_idx;         /* graph: neighbour j, or ~j when the cell is
+                                   the second cell of the connection */
+    const R* csr_g;             /* graph: conductance per entry */
+    const R* gx_field;          /* hetero: (ny, nx - 1) */
+    const R* gy_field;          /* hetero: (ny - 1, nx) */
+    const long long* dom_off;   /* block driver: domain offsets into dom_cells */
+    const int* dom_cells;       /* block driver: cell ids, grouped by domain
+                                   (NULL = identity order) */
+    int* halt;                  /* halt[0]: log row where NaN was found */
+    long long n;                /* number of cells */
+    long long npad;             /* padded cell count (SoA stride) */
+    long long nx, ny;           /* grid size */
+    long long px0, px1, py0, py1;   /* paced rectangle [px0,px1) x [py0,py1) */
+    int pace_mode;              /* 0 all cells, 1 rectangle, 2 mask, 3 none */
+    int ncols;                  /* per-cell log columns */
+    int nan_check;              /* test state 0 of cell 0 at log steps */
+    int ndom;                   /* number of domains (block driver) */
+    R gx, gy;                   /* homogeneous conductances */
+};
+
+#endif

@@ -1,0 +1,1195 @@
+/*
+ * mkh_host.cpp -- host side of the MI355X tissue engine (C ABI in
+ * include/myokit_amd.h).
+ *
+ * Replaces the generated OpenCL host module of myokit/_sim/openclsim.c:
+ *   - the kernel program is compiled once per generated source with hiprtc
+ *     for the device's gfx target and cached (instead of clBuildProgram on
+ *     every run, openclsim.c:815-839);
+ *   - the time-step rule of sim_step (openclsim.c:1051-1162) runs on the
+ *     host as a planner that writes a step table (time, dt, pace, log row)
+ *     for a whole batch of steps; the device then runs the batch without
+ *     host round trips, instead of 2-3 kernel enqueues plus blocking reads
+ *     per step;
+ *   - state, fields and logs stay resident in device memory (SoA), the log
+ *     is copied back once per batch in bulk.
+ */
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <unistd.h>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/myokit_amd.h"
+#include "mk_args.h"
+#include "mk_esys.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define MKH_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            return fail(MKH_E_HIP, std::string(#call) + " failed: " +        \
+                                       hipGetErrorString(e_));               \
+    } while (0)
+
+/* -------------------------------------------------------------------------
+ * Program: compiled kernel module
+ * ---------------------------------------------------------------------- */
+struct Program {
+    int device = -1;
+    hipModule_t module = nullptr;
+    hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
+    std::vector<char> code;
+};
+
+unsigned long long fnv1a(const std::string& s, unsigned long long h)
+{
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+std::vector<std::string> split_options(const char* options)
+{
+    std::vector<std::string> out;
+    if (!options) return out;
+    std::istringstream is(options);
+    std::string w;
+    while (is >> w) out.push_back(w);
+    return out;
+}
+
+/* -------------------------------------------------------------------------
+ * Time-step planner: the step and log rule of sim_step
+ * (openclsim.c:1051-1162), with dt_min = 0.
+ * ---------------------------------------------------------------------- */
+struct StepRec {
+    double t, dt, pace, t_after;
+    bool log;
+};
+
+struct Planner {
+    double tmin = 0, tmax = 0, dt0 = 0, log_interval = 0;
+    double t = 0, tnext_log = 0, tnext_pace = 0, pace = 0;
+    unsigned long long istep = 1, inext_log = 0;
+    bool done = false;
+    MkESys esys;
+
+    int init(double tmin_, double tmax_, double dt_, const double* events,
+             long long n_events, double log_interval_)
+    {
+        tmin = tmin_;
+        tmax = tmax_;
+        dt0 = dt_;
+        log_interval = log_interval_;
+        int f = esys.init(tmin, events, n_events);
+        if (f != MkESys::OK) return f;
+        tnext_pace = esys.next_time();
+        pace = esys.level();
+        t = tmin;
+        istep = 1;
+        inext_log = 0;
+        tnext_log = tmin;
+        done = !(tmax > tmin);
+        return 0;
+    }
+
+    int next(StepRec& r)
+    {
+        const bool logging = t >= tnext_log;
+        bool inter = false;
+        double dt = tmin + (double)istep * dt0 - t;
+        double d = tmax - t;
+        if (d > 0 && d < dt) { dt = d; inter = true; }
+        d = tnext_pace - t;
+        if (d > 0 && d < dt) { dt = d; inter = true; }
+        d = tnext_log - t;
+        if (d > 0 && d < dt) { dt = d; inter = true; }
+        if (!inter) istep++;
+        r.t = t;
+        r.dt = dt;
+        r.pace = pace;
+        r.log = logging;
+        if (logging) {
+            inext_log++;
+            tnext_log = tmin + (double)inext_log * log_interval;
+        }
+        t += dt;
+        r.t_after = t;
+        int f = esys.advance(t);
+        if (f != MkESys::OK) return f;
+        tnext_pace = esys.next_time();
+        pace = esys.level();
+        if (t >= tmax) done = true;
+        return 0;
+    }
+};
+
+int pacing_error(int f)
+{
+    if (f == MkESys::SIMULTANEOUS)
+        return fail(MKH_E_PACING_SIMULTANEOUS,
+                    "E-Pacing error: Event scheduled or re-occuring at the "
+                    "same time as another event.");
+    if (f == MkESys::NEGATIVE_TIME)
+        return fail(MKH_E_PACING_NEGATIVE_TIME,
+                    "E-Pacing error: New time is before current time.");
+    return fail(MKH_E_PACING_INVALID, "E-Pacing error: invalid protocol.");
+}
+
+/* -------------------------------------------------------------------------
+ * Simulation
+ * ---------------------------------------------------------------------- */
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (count == n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+};
+
+struct SimBase {
+    virtual ~SimBase() {}
+    mkh_sim_config cfg{};
+    Program* prog = nullptr;
+    virtual int set_state(const double*, long long) = 0;
+    virtual int get_state(double*, long long) = 0;
+    virtual int set_fields(const double*, long long) = 0;
+    virtual int set_conductance(double, double) = 0;
+    virtual int set_conductance_field(const double*, long long, const double*,
+                                      long long) = 0;
+    virtual int set_connections(const long long*, const long long*,
+                                const double*, long long) = 0;
+    virtual int set_domains(const long long*, long long) = 0;
+    virtual int set_pacing(int mode, const long long* a, long long n) = 0;
+    virtual int set_logging(const int*, long long, int, int) = 0;
+    virtual int init(double, double, double, const double*, long long, double,
+                     long long*) = 0;
+    virtual int set_log(void*, double*, double*) = 0;
+    virtual int step(double*, int*) = 0;
+    virtual int clean() = 0;
+    virtual int set_option(const std::string&, long long) = 0;
+    virtual int timing(double*, long long*, long long*) = 0;
+    virtual int reset_timing() = 0;
+    virtual long long log_rows() = 0;
+};
+
+template <typename R>
+struct Sim : SimBase {
+    typedef MkArgsT<R> Args;
+    typedef MkStepT<R> Step;
+
+    long long n = 0, npad = 0;
+    int cur = 0;                  /* state buffer holding the current state */
+    hipStream_t stream = nullptr;
+    DevBuf<R> state[2], fields, csr_g, gxf, gyf, logbuf;
+    DevBuf<int> colmap, csr_ptr, csr_idx, dom_cells, halt;
+    DevBuf<long long> dom_off;
+    DevBuf<unsigned char> pace_mask;
+    DevBuf<Step> steps;
+    R gx = 0, gy = 0;
+    int pace_mode = 0;
+    long long px0 = 0, px1 = 0, py0 = 0, py1 = 0;
+    int ncols = 0, nan_check = 0;
+    long long nquant = 0;
+    int driver = MKH_DRIVER_GRID;
+    long long ndom = 1;
+    long long max_domain = 0;
+    int max_block_cells = 0;
+
+    /* graph connections kept on the host for re-layout */
+    std::vector<long long> ci, cj;
+    std::vector<double> cg;
+    bool have_graph = false;
+
+    /* options */
+    long long batch_steps = 1 << 16;
+    long long log_rows_cap = 0;   /* 0 = automatic */
+    bool time_kernels = false;
+    int forced_driver = -1;
+
+    /* run state */
+    bool running = false;
+    Planner plan;
+    long long total_rows = 0, rows_done = 0;
+    R* dst_values = nullptr;
+    double* dst_times = nullptr;
+    double* dst_paces = nullptr;
+    Step* h_steps = nullptr;       /* pinned staging */
+    size_t h_steps_cap = 0;
+    std::vector<double> row_t_after;
+
+    /* timing */
+    double kernel_ms = 0;
+    long long launches = 0, cell_steps = 0;
+    std::vector<hipEvent_t> ev_pool;
+
+    ~Sim() override
+    {
+        if (h_steps) (void)hipHostFree(h_steps);
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int create(Program* p, const mkh_sim_config* c)
+    {
+        prog = p;
+        cfg = *c;
+        n = cfg.nx * cfg.ny;
+        npad = (n + 255) / 256 * 256;
+        max_block_cells = cfg.precision == 64 ? 2048 : 4096;
+        MKH_HIP(hipSetDevice(cfg.device));
+        MKH_HIP(hipStreamCreate(&stream));
+        MKH_HIP(state[0].alloc((size_t)cfg.n_state * npad));
+        MKH_HIP(state[1].alloc((size_t)cfg.n_state * npad));
+        MKH_HIP(hipMemset(state[0].p, 0, state[0].n * sizeof(R)));
+        MKH_HIP(hipMemset(state[1].p, 0, state[1].n * sizeof(R)));
+        if (cfg.n_field > 0) {
+            MKH_HIP(fields.alloc((size_t)cfg.n_field * npad));
+            MKH_HIP(hipMemset(fields.p, 0, fields.n * sizeof(R)));
+        }
+        MKH_HIP(halt.alloc(1));
+        int m1 = -1;
+        MKH_HIP(hipMemcpy(halt.p, &m1, sizeof(int), hipMemcpyHostToDevice));
+        long long off[2] = {0, n};
+        MKH_HIP(dom_off.alloc(2));
+        MKH_HIP(hipMemcpy(dom_off.p, off, sizeof(off), hipMemcpyHostToDevice));
+        ndom = 1;
+        max_domain = n;
+        pace_mode = 0;
+        choose_driver();
+        return 0;
+    }
+
+    void choose_driver()
+    {
+        if (forced_driver >= 0) {
+            driver = forced_driver;
+            return;
+        }
+        if (cfg.coupling == MKH_COUPLING_NONE) {
+            driver = MKH_DRIVER_CELLS;
+        } else if (max_domain <= max_block_cells &&
+                   (ndom >= 64 || max_domain <= 512)) {
+            driver = MKH_DRIVER_BLOCK;
+        } else {
+            driver = MKH_DRIVER_GRID;
+        }
+    }
+
+    int set_state(const double* s, long long count) override
+    {
+        if (count != n * cfg.n_state)
+            return fail(MKH_E_ARG, "state has the wrong size");
+        std::vector<R> soa((size_t)cfg.n_state * npad, (R)0);
+        for (long long c = 0; c < n; ++c)
+            for (int i = 0; i < cfg.n_state; ++i)
+                soa[(size_t)i * npad + c] = (R)s[c * cfg.n_state + i];
+        MKH_HIP(hipMemcpy(state[cur].p, soa.data(), soa.size() * sizeof(R),
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
+
+    int get_state(double* s, long long count) override
+    {
+        if (count != n * cfg.n_state)
+            return fail(MKH_E_ARG, "state has the wrong size");
+        std::vector<R> soa((size_t)cfg.n_state * npad);
+        MKH_HIP(hipStreamSynchronize(stream));
+        MKH_HIP(hipMemcpy(soa.data(), state[cur].p, soa.size() * sizeof(R),
+                          hipMemcpyDeviceToHost));
+        for (long long c = 0; c < n; ++c)
+            for (int i = 0; i < cfg.n_state; ++i)
+                s[c * cfg.n_state + i] = (double)soa[(size_t)i * npad + c];
+        return 0;
+    }
+
+    int set_fields(const double* d, long long count) override
+    {
+        if (count != n * cfg.n_field)
+            return fail(MKH_E_ARG, "field data has the wrong size");
+        if (cfg.n_field == 0) return 0;
+        std::vector<R> soa((size_t)cfg.n_field * npad, (R)0);
+        for (long long c = 0; c < n; ++c)
+            for (int k = 0; k < cfg.n_field; ++k)
+                soa[(size_t)k * npad + c] = (R)d[c * cfg.n_field + k];
+        MKH_HIP(hipMemcpy(fields.p, soa.data(), soa.size() * sizeof(R),
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
+
+    int set_conductance(double x, double y) override
+    {
+        gx = (R)x;
+        gy = (R)y;
+        return 0;
+    }
+
+    int set_conductance_field(const double* x, long long nxf,
+                              const double* y, long long nyf) override
+    {
+        if (nxf != (cfg.nx - 1) * cfg.ny)
+            return fail(MKH_E_ARG, "gx field has the wrong size");
+        if (cfg.ny > 1 && nyf != (cfg.ny - 1) * cfg.nx)
+            return fail(MKH_E_ARG, "gy field has the wrong size");
+        std::vector<R> hx(std::max<long long>(nxf, 1), (R)0);
+        for (long long k = 0; k < nxf; ++k) hx[k] = (R)x[k];
+        MKH_HIP(gxf.alloc(hx.size()));
+        MKH_HIP(hipMemcpy(gxf.p, hx.data(), hx.size() * sizeof(R),
+                          hipMemcpyHostToDevice));
+        std::vector<R> hy(std::max<long long>(nyf, 1), (R)0);
+        for (long long k = 0; k < nyf && y; ++k) hy[k] = (R)y[k];
+        MKH_HIP(gyf.alloc(hy.size()));
+        MKH_HIP(hipMemcpy(gyf.p, hy.data(), hy.size() * sizeof(R),
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
+
+    int set_connections(const long long* i, const long long* j,
+                        const double* g, long long m) override
+    {
+        ci.assign(i, i + m);
+        cj.assign(j, j + m);
+        cg.assign(g, g + m);
+        for (long long k = 0; k < m; ++k) {
+            if (ci[k] < 0 || ci[k] >= n || cj[k] < 0 || cj[k] >= n ||
+                ci[k] == cj[k])
+                return fail(MKH_E_ARG, "invalid connection");
+        }
+        have_graph = true;
+        /* Connected components (union-find) */
+        std::vector<long long> parent(n);
+        std::iota(parent.begin(), parent.end(), 0);
+        auto find = [&](long long a) {
+            while (parent[a] != a) {
+                parent[a] = parent[parent[a]];
+                a = parent[a];
+            }
+            return a;
+        };
+        for (long long k = 0; k < m; ++k) {
+            long long a = find(ci[k]), b = find(cj[k]);
+            if (a != b) parent[std::max(a, b)] = std::min(a, b);
+        }
+        std::vector<long long> root(n), comp_size(n, 0);
+        for (long long c = 0; c < n; ++c) {
+            root[c] = find(c);
+            comp_size[root[c]]++;
+        }
+        max_domain = 0;
+        for (long long c = 0; c < n; ++c)
+            max_domain = std::max(max_domain, comp_size[c]);
+        choose_driver_graph();
+        return build_graph_layout(root, comp_size);
+    }
+
+    void choose_driver_graph()
+    {
+        ndom = 0;   /* set by build_graph_layout */
+        if (forced_driver >= 0) {
+            driver = forced_driver;
+            return;
+        }
+        driver = max_domain <= max_block_cells ? MKH_DRIVER_BLOCK
+                                               : MKH_DRIVER_GRID;
+    }
+
+    int build_graph_layout(const std::vector<long long>& root,
+                           const std::vector<long long>& comp_size)
+    {
+        const long long m = (long long)ci.size();
+        if (driver == MKH_DRIVER_BLOCK) {
+            /* Domains = components, cells in increasing id order; CSR rows in
+             * domain order with neighbour indices local to the domain. */
+            std::vector<long long> order(n);
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](long long a, long long b) {
+                                 return root[a] < root[b];
+                             });
+            std::vector<long long> pos(n), offs;
+            std::vector<long long> dstart(n, -1);
+            for (long long p = 0; p < n; ++p) {
+                long long c = order[p];
+                if (p == 0 || root[order[p - 1]] != root[c]) {
+                    offs.push_back(p);
+                    dstart[root[c]] = p;
+                }
+                pos[c] = p;
+            }
+            offs.push_back(n);
+            ndom = (long long)offs.size() - 1;
+            std::vector<std::vector<std::pair<int, R>>> rows(n);
+            for (long long k = 0; k < m; ++k) {
+                long long a = ci[k], b = cj[k];
+                long long base = dstart[root[a]];
+                rows[pos[a]].push_back({(int)(pos[b] - base), (R)cg[k]});
+                rows[pos[b]].push_back({~(int)(pos[a] - base), (R)cg[k]});
+            }
+            std::vector<int> hptr(n + 1, 0), hidx, hcells(n);
+            std::vector<R> hg;
+            for (long long p = 0; p < n; ++p) {
+                hptr[p + 1] = hptr[p] + (int)rows[p].size();
+                for (auto& e : rows[p]) {
+                    hidx.push_back(e.first);
+                    hg.push_back(e.second);
+                }
+                hcells[p] = (int)order[p];
+            }
+            if (hidx.empty()) {
+                hidx.push_back(0);
+                hg.push_back((R)0);
+            }
+            MKH_HIP(csr_ptr.alloc(hptr.size()));
+            MKH_HIP(hipMemcpy(csr_ptr.p, hptr.data(), hptr.size() * 4,
+                              hipMemcpyHostToDevice));
+            MKH_HIP(csr_idx.alloc(hidx.size()));
+            MKH_HIP(hipMemcpy(csr_idx.p, hidx.data(), hidx.size() * 4,
+                              hipMemcpyHostToDevice));
+            MKH_HIP(csr_g.alloc(hg.size()));
+            MKH_HIP(hipMemcpy(csr_g.p, hg.data(), hg.size() * sizeof(R),
+                              hipMemcpyHostToDevice));
+            MKH_HIP(dom_cells.alloc(hcells.size()));
+            MKH_HIP(hipMemcpy(dom_cells.p, hcells.data(), hcells.size() * 4,
+                              hipMemcpyHostToDevice));
+            MKH_HIP(dom_off.alloc(offs.size()));
+            MKH_HIP(hipMemcpy(dom_off.p, offs.data(),
+                              offs.size() * sizeof(long long),
+                              hipMemcpyHostToDevice));
+        } else {
+            /* Global CSR by cell id, entries in connection-list order. */
+            std::vector<int> cnt(n + 1, 0);
+            for (long long k = 0; k < m; ++k) {
+                cnt[ci[k] + 1]++;
+                cnt[cj[k] + 1]++;
+            }
+            for (long long c = 0; c < n; ++c) cnt[c + 1] += cnt[c];
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+            std::vector<int> hidx(std::max<long long>(2 * m, 1), 0);
+            std::vector<R> hg(std::max<long long>(2 * m, 1), (R)0);
+            for (long long k = 0; k < m; ++k) {
+                int a = (int)ci[k], b = (int)cj[k];
+                hidx[fill[a]] = b;
+                hg[fill[a]++] = (R)cg[k];
+                hidx[fill[b]] = ~a;
+                hg[fill[b]++] = (R)cg[k];
+            }
+            MKH_HIP(csr_ptr.alloc(cnt.size()));
+            MKH_HIP(hipMemcpy(csr_ptr.p, cnt.data(), cnt.size() * 4,
+                              hipMemcpyHostToDevice));
+            MKH_HIP(csr_idx.alloc(hidx.size()));
+            MKH_HIP(hipMemcpy(csr_idx.p, hidx.data(), hidx.size() * 4,
+                              hipMemcpyHostToDevice));
+            MKH_HIP(csr_g.alloc(hg.size()));
+            MKH_HIP(hipMemcpy(csr_g.p, hg.data(), hg.size() * sizeof(R),
+                              hipMemcpyHostToDevice));
+            ndom = 1;
+        }
+        (void)comp_size;
+        return 0;
+    }
+
+    int set_domains(const long long* offs, long long nd) override
+    {
+        if (nd < 1 || offs[0] != 0 || offs[nd] != n)
+            return fail(MKH_E_ARG, "domain offsets must run from 0 to n");
+        max_domain = 0;
+        for (long long d = 0; d < nd; ++d) {
+            if (offs[d + 1] < offs[d])
+                return fail(MKH_E_ARG, "domain offsets must increase");
+            max_domain = std::max(max_domain, offs[d + 1] - offs[d]);
+        }
+        MKH_HIP(dom_off.alloc(nd + 1));
+        MKH_HIP(hipMemcpy(dom_off.p, offs, (nd + 1) * sizeof(long long),
+                          hipMemcpyHostToDevice));
+        ndom = nd;
+        dom_cells.release();
+        choose_driver();
+        return 0;
+    }
+
+    int set_pacing(int mode, const long long* a, long long m) override
+    {
+        pace_mode = mode;
+        if (mode == 1) {
+            /* rectangle (nx, ny, x, y) as normalised by set_paced_cells */
+            px0 = a[2];
+            px1 = a[2] + a[0];
+            py0 = a[3];
+            py1 = a[3] + a[1];
+        } else if (mode == 2) {
+            std::vector<unsigned char> mask(npad, 0);
+            for (long long k = 0; k < m; ++k) {
+                if (a[k] < 0 || a[k] >= n)
+                    return fail(MKH_E_ARG, "paced cell out of range");
+                mask[a[k]] = 1;
+            }
+            MKH_HIP(pace_mask.alloc(npad));
+            MKH_HIP(hipMemcpy(pace_mask.p, mask.data(), npad,
+                              hipMemcpyHostToDevice));
+        }
+        return 0;
+    }
+
+    int set_logging(const int* cmap, long long nq, int nc, int nanc) override
+    {
+        if (nq != cfg.n_quant)
+            return fail(MKH_E_ARG, "quantity count differs from the kernel");
+        nquant = nq;
+        ncols = nc;
+        nan_check = nanc;
+        if (nq > 0) {
+            std::vector<int> h((size_t)nq * npad, -1);
+            for (long long q = 0; q < nq; ++q)
+                for (long long c = 0; c < n; ++c)
+                    h[(size_t)q * npad + c] = cmap[q * n + c];
+            MKH_HIP(colmap.alloc(h.size()));
+            MKH_HIP(hipMemcpy(colmap.p, h.data(), h.size() * sizeof(int),
+                              hipMemcpyHostToDevice));
+        }
+        return 0;
+    }
+
+    int init(double tmin, double tmax, double dt, const double* events,
+             long long nev, double log_interval, long long* nrows) override
+    {
+        if (!(dt > 0)) return fail(MKH_E_ARG, "step size must be positive");
+        if (!(log_interval > 0))
+            return fail(MKH_E_ARG, "log interval must be positive");
+        int f = plan.init(tmin, tmax, dt, events, nev, log_interval);
+        if (f) return pacing_error(f);
+        /* Dry run to count the log rows */
+        Planner dry = plan;
+        long long rows = 0;
+        StepRec r;
+        while (!dry.done) {
+            f = dry.next(r);
+            if (f) return pacing_error(f);
+            if (r.log) rows++;
+        }
+        total_rows = rows;
+        rows_done = 0;
+        if (nrows) *nrows = rows;
+        dst_values = nullptr;
+        dst_times = dst_paces = nullptr;
+        int m1 = -1;
+        MKH_HIP(hipMemcpy(halt.p, &m1, sizeof(int), hipMemcpyHostToDevice));
+        /* Device log buffer: whole run if it fits a budget, else batches */
+        long long cap = log_rows_cap;
+        if (cap <= 0) {
+            const long long budget = 1LL << 30;  /* bytes */
+            long long row_bytes = std::max<long long>(1, ncols) * sizeof(R);
+            cap = std::max<long long>(1, budget / row_bytes);
+        }
+        cap = std::max<long long>(1, std::min<long long>(cap, rows));
+        if (ncols > 0 && rows > 0) {
+            MKH_HIP(logbuf.alloc((size_t)cap * ncols));
+        }
+        log_rows_cap_run = cap;
+        size_t need = (size_t)batch_steps;
+        MKH_HIP(steps.alloc(need));
+        if (h_steps_cap < need) {
+            if (h_steps) (void)hipHostFree(h_steps);
+            MKH_HIP(hipHostMalloc((void**)&h_steps, need * sizeof(Step)));
+            h_steps_cap = need;
+        }
+        running = true;
+        return 0;
+    }
+    long long log_rows_cap_run = 1;
+
+    int set_log(void* v, double* t, double* p) override
+    {
+        dst_values = (R*)v;
+        dst_times = t;
+        dst_paces = p;
+        return 0;
+    }
+
+    Args base_args()
+    {
+        Args a;
+        std::memset(&a, 0, sizeof(a));
+        a.state_a = state[cur].p;
+        a.state_b = state[cur ^ 1].p;
+        a.fields = fields.p;
+        a.steps = steps.p;
+        a.logbuf = logbuf.p;
+        a.colmap = colmap.p;
+        a.pace_mask = pace_mask.p;
+        a.csr_ptr = csr_ptr.p;
+        a.csr_idx = csr_idx.p;
+        a.csr_g = csr_g.p;
+        a.gx_field = gxf.p;
+        a.gy_field = gyf.p;
+        a.dom_off = dom_off.p;
+        a.dom_cells = dom_cells.p;
+        a.halt = halt.p;
+        a.n = n;
+        a.npad = npad;
+        a.nx = cfg.nx;
+        a.ny = cfg.ny;
+        a.px0 = px0;
+        a.px1 = px1;
+        a.py0 = py0;
+        a.py1 = py1;
+        a.pace_mode = pace_mode;
+        a.ncols = ncols;
+        a.nan_check = nan_check;
+        a.ndom = (int)ndom;
+        a.gx = gx;
+        a.gy = gy;
+        return a;
+    }
+
+    hipEvent_t event(size_t k)
+    {
+        while (ev_pool.size() <= k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev_pool.push_back(e);
+        }
+        return ev_pool[k];
+    }
+
+    int launch(hipFunction_t f, unsigned grid, void** params, size_t& nev)
+    {
+        const unsigned bt = (unsigned)cfg.block_threads;
+        if (time_kernels) MKH_HIP(hipEventRecord(event(nev), stream));
+        MKH_HIP(hipModuleLaunchKernel(f, grid, 1, 1, bt, 1, 1, 0, stream,
+                                      params, nullptr));
+        if (time_kernels) MKH_HIP(hipEventRecord(event(nev + 1), stream));
+        if (time_kernels) nev += 2;
+        launches++;
+        return 0;
+    }
+
+    int step(double* tout, int* status) override
+    {
+        if (!running) return fail(MKH_E_STATE, "simulation not initialised");
+        /* Plan one batch */
+        long long ns = 0, nr = 0;
+        row_t_after.clear();
+        std::vector<double> tlog, plog;
+        StepRec r;
+        while (!plan.done && ns < batch_steps && nr < log_rows_cap_run) {
+            int f = plan.next(r);
+            if (f) return pacing_error(f);
+            Step& s = h_steps[ns++];
+            s.t = (R)r.t;
+            s.dt = (R)r.dt;
+            s.pace = (R)r.pace;
+            s.row = -1;
+            if (r.log) {
+                s.row = (int)nr++;
+                row_t_after.push_back(r.t_after);
+                tlog.push_back((double)(R)r.t);
+                plog.push_back((double)(R)r.pace);
+            }
+        }
+        if (ns == 0) {
+            *tout = plan.t;
+            *status = 1;
+            return 0;
+        }
+        MKH_HIP(hipMemcpyAsync(steps.p, h_steps, ns * sizeof(Step),
+                               hipMemcpyHostToDevice, stream));
+        /* Launch */
+        size_t nev = 0;
+        const unsigned bt = (unsigned)cfg.block_threads;
+        if (driver == MKH_DRIVER_GRID) {
+            unsigned grid;
+            if ((cfg.coupling == MKH_COUPLING_STENCIL ||
+                 cfg.coupling == MKH_COUPLING_HETERO) && cfg.ny > 1) {
+                long long tx = (cfg.nx + 63) / 64;
+                long long ty = (cfg.ny + (bt / 64) - 1) / (bt / 64);
+                grid = (unsigned)(tx * ty);
+            } else {
+                grid = (unsigned)((n + bt - 1) / bt);
+            }
+            for (long long s = 0; s < ns; ++s) {
+                Args a = base_args();
+                R* src = state[cur].p;
+                R* dst = state[cur ^ 1].p;
+                int si = (int)s;
+                void* params[] = {&a, &src, &dst, &si};
+                int e = launch(prog->f_grid, grid, params, nev);
+                if (e) return e;
+                cur ^= 1;
+            }
+        } else {
+            hipFunction_t f = driver == MKH_DRIVER_CELLS ? prog->f_cells
+                                                         : prog->f_block;
+            unsigned grid = driver == MKH_DRIVER_CELLS
+                                ? (unsigned)((n + bt - 1) / bt)
+                                : (unsigned)ndom;
+            Args a = base_args();
+            long long s0 = 0;
+            for (long long s = 0; s < ns; ++s) {
+                if (h_steps[s].row >= 0 || s == ns - 1) {
+                    int i0 = (int)s0, i1 = (int)(s + 1);
+                    void* params[] = {&a, &i0, &i1};
+                    int e = launch(f, grid, params, nev);
+                    if (e) return e;
+                    s0 = s + 1;
+                }
+            }
+        }
+        cell_steps += ns * n;
+        /* Bring back the log and the halt flag */
+        int h = -1;
+        MKH_HIP(hipMemcpyAsync(&h, halt.p, sizeof(int), hipMemcpyDeviceToHost,
+                               stream));
+        MKH_HIP(hipStreamSynchronize(stream));
+        long long valid = nr;
+        if (h >= 0) valid = h + 1;
+        if (nr > 0 && ncols > 0 && dst_values)
+            MKH_HIP(hipMemcpy(dst_values + (size_t)rows_done * ncols,
+                              logbuf.p, (size_t)valid * ncols * sizeof(R),
+                              hipMemcpyDeviceToHost));
+        for (long long k = 0; k < valid; ++k) {
+            if (dst_times) dst_times[rows_done + k] = tlog[k];
+            if (dst_paces) dst_paces[rows_done + k] = plog[k];
+        }
+        rows_done += valid;
+        if (time_kernels) {
+            for (size_t k = 0; k + 1 < nev; k += 2) {
+                float ms = 0;
+                MKH_HIP(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
+                kernel_ms += ms;
+            }
+        }
+        if (h >= 0) {
+            *tout = row_t_after[h];
+            *status = 2;
+            running = false;
+            return fail(MKH_E_NAN, "Encountered nan in simulation.");
+        }
+        *tout = plan.t;
+        *status = plan.done ? 1 : 0;
+        if (plan.done) running = false;
+        return 0;
+    }
+
+    int clean() override
+    {
+        running = false;
+        return 0;
+    }
+
+    int set_option(const std::string& k, long long v) override
+    {
+        if (k == "batch_steps") {
+            if (v < 1) return fail(MKH_E_ARG, "batch_steps must be >= 1");
+            batch_steps = v;
+        } else if (k == "log_rows") {
+            log_rows_cap = v;
+        } else if (k == "time_kernels") {
+            time_kernels = v != 0;
+        } else if (k == "driver") {
+            forced_driver = (int)v;
+            if (have_graph) {
+                driver = (int)v;
+            } else {
+                choose_driver();
+            }
+        } else {
+            return fail(MKH_E_ARG, "unknown option " + k);
+        }
+        return 0;
+    }
+
+    int timing(double* ms, long long* nl, long long* cs) override
+    {
+        if (ms) *ms = kernel_ms;
+        if (nl) *nl = launches;
+        if (cs) *cs = cell_steps;
+        return 0;
+    }
+
+    int reset_timing() override
+    {
+        kernel_ms = 0;
+        launches = 0;
+        cell_steps = 0;
+        return 0;
+    }
+
+    long long log_rows() override { return rows_done; }
+};
+
+}  // namespace
+
+/* -------------------------------------------------------------------------
+ * C ABI
+ * ---------------------------------------------------------------------- */
+extern "C" {
+
+const char* mkh_version(void) { return "myokit_amd 0.1"; }
+
+const char* mkh_last_error(void) { return g_err.c_str(); }
+
+int mkh_device_count(int* n)
+{
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    if (n) *n = c;
+    return 0;
+}
+
+int mkh_device_info(int device, char* name, int name_len,
+                    long long* total_mem, int* n_cu)
+{
+    hipDeviceProp_t p;
+    MKH_HIP(hipGetDeviceProperties(&p, device));
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (total_mem) *total_mem = (long long)p.totalGlobalMem;
+    if (n_cu) *n_cu = p.multiProcessorCount;
+    return 0;
+}
+
+int mkh_program_build(const char* source, const char* options,
+                      const char* cache_dir, int device, void** program,
+                      char* build_log, int build_log_len)
+{
+    if (!source || !program) return fail(MKH_E_ARG, "null argument");
+    std::string arch = "gfx950";
+    if (device >= 0) {
+        hipDeviceProp_t p;
+        MKH_HIP(hipGetDeviceProperties(&p, device));
+        arch = p.gcnArchName;
+        size_t colon = arch.find(':');
+        if (colon != std::string::npos) arch = arch.substr(0, colon);
+    }
+    std::vector<std::string> opts = split_options(options);
+    opts.push_back("--offload-arch=" + arch);
+    std::string key = std::string(source);
+    for (auto& o : opts) key += "\x01" + o;
+    unsigned long long h = fnv1a(key, 1469598103934665603ULL);
+    char hex[32];
+    std::snprintf(hex, sizeof(hex), "%016llx", h);
+    std::string cpath;
+    if (cache_dir && cache_dir[0])
+        cpath = std::string(cache_dir) + "/mk_" + hex + "_" + arch + ".co";
+
+    std::unique_ptr<Program> prog(new Program());
+    prog->device = device;
+    bool have = false;
+    if (!cpath.empty()) {
+        std::ifstream f(cpath, std::ios::binary);
+        if (f) {
+            prog->code.assign(std::istreambuf_iterator<char>(f),
+                              std::istreambuf_iterator<char>());
+            have = !prog->code.empty();
+        }
+    }
+    if (!have) {
+        hiprtcProgram rp;
+        hiprtcResult r =
+            hiprtcCreateProgram(&rp, source, "mk_kernel.hip", 0, nullptr,
+                                nullptr);
+        if (r != HIPRTC_SUCCESS)
+            return fail(MKH_E_COMPILE, hiprtcGetErrorString(r));
+        std::vector<const char*> cop;
+        for (auto& o : opts) cop.push_back(o.c_str());
+        r = hiprtcCompileProgram(rp, (int)cop.size(), cop.data());
+        size_t lsz = 0;
+        hiprtcGetProgramLogSize(rp, &lsz);
+        std::string log(lsz, '\0');
+        if (lsz) hiprtcGetProgramLog(rp, &log[0]);
+        if (build_log && build_log_len > 0) {
+            std::snprintf(build_log, build_log_len, "%s", log.c_str());
+        }
+        if (r != HIPRTC_SUCCESS) {
+            hiprtcDestroyProgram(&rp);
+            return fail(MKH_E_COMPILE,
+                        std::string("hiprtc compile failed: ") +
+                            hiprtcGetErrorString(r) + "\n" + log);
+        }
+        size_t csz = 0;
+        hiprtcGetCodeSize(rp, &csz);
+        prog->code.resize(csz);
+        hiprtcGetCode(rp, prog->code.data());
+        hiprtcDestroyProgram(&rp);
+        if (!cpath.empty()) {
+            std::string tmp = cpath + ".tmp" + std::to_string((long long)getpid());
+            std::ofstream f(tmp, std::ios::binary);
+            if (f) {
+                f.write(prog->code.data(), (std::streamsize)prog->code.size());
+                f.close();
+                std::rename(tmp.c_str(), cpath.c_str());
+            }
+        }
+    }
+    if (device >= 0) {
+        MKH_HIP(hipSetDevice(device));
+        MKH_HIP(hipModuleLoadData(&prog->module, prog->code.data()));
+        MKH_HIP(hipModuleGetFunction(&prog->f_cells, prog->module, "mk_cells"));
+        MKH_HIP(hipModuleGetFunction(&prog->f_block, prog->module, "mk_block"));
+        MKH_HIP(hipModuleGetFunction(&prog->f_grid, prog->module, "mk_grid"));
+    }
+    *program = prog.release();
+    return 0;
+}
+
+int mkh_program_free(void* program)
+{
+    Program* p = (Program*)program;
+    if (!p) return 0;
+    if (p->module) (void)hipModuleUnload(p->module);
+    delete p;
+    return 0;
+}
+
+int mkh_sim_create(void* program, const mkh_sim_config* cfg, void** sim)
+{
+    if (!program || !cfg || !sim) return fail(MKH_E_ARG, "null argument");
+    Program* p = (Program*)program;
+    if (!p->module)
+        return fail(MKH_E_STATE, "program was compiled without a device");
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->n_state < 1)
+        return fail(MKH_E_ARG, "invalid geometry");
+    if (cfg->block_threads < 64 || cfg->block_threads > 1024 ||
+        cfg->block_threads % 64)
+        return fail(MKH_E_ARG, "block_threads must be a multiple of 64");
+    if (cfg->precision != 32 && cfg->precision != 64)
+        return fail(MKH_E_ARG, "precision must be 32 or 64");
+    SimBase* s;
+    int r;
+    if (cfg->precision == 64) {
+        Sim<double>* d = new Sim<double>();
+        r = d->create(p, cfg);
+        s = d;
+    } else {
+        Sim<float>* d = new Sim<float>();
+        r = d->create(p, cfg);
+        s = d;
+    }
+    if (r) {
+        delete s;
+        return r;
+    }
+    *sim = s;
+    return 0;
+}
+
+#define SIM ((SimBase*)sim)
+#define CHECK_SIM() \
+    if (!sim) return fail(MKH_E_ARG, "null simulation handle")
+
+int mkh_sim_destroy(void* sim)
+{
+    if (sim) delete SIM;
+    return 0;
+}
+
+int mkh_sim_set_state(void* sim, const double* s, long long n)
+{
+    CHECK_SIM();
+    return SIM->set_state(s, n);
+}
+
+int mkh_sim_get_state(void* sim, double* s, long long n)
+{
+    CHECK_SIM();
+    return SIM->get_state(s, n);
+}
+
+int mkh_sim_set_fields(void* sim, const double* d, long long n)
+{
+    CHECK_SIM();
+    return SIM->set_fields(d, n);
+}
+
+int mkh_sim_set_conductance(void* sim, double gx, double gy)
+{
+    CHECK_SIM();
+    return SIM->set_conductance(gx, gy);
+}
+
+int mkh_sim_set_conductance_field(void* sim, const double* gx, long long ngx,
+                                  const double* gy, long long ngy)
+{
+    CHECK_SIM();
+    return SIM->set_conductance_field(gx, ngx, gy, ngy);
+}
+
+int mkh_sim_set_connections(void* sim, const long long* i,
+                            const long long* j, const double* g, long long n)
+{
+    CHECK_SIM();
+    return SIM->set_connections(i, j, g, n);
+}
+
+int mkh_sim_set_domains(void* sim, const long long* offsets, long long nd)
+{
+    CHECK_SIM();
+    return SIM->set_domains(offsets, nd);
+}
+
+int mkh_sim_set_pacing_all(void* sim)
+{
+    CHECK_SIM();
+    return SIM->set_pacing(0, nullptr, 0);
+}
+
+int mkh_sim_set_pacing_none(void* sim)
+{
+    CHECK_SIM();
+    return SIM->set_pacing(3, nullptr, 0);
+}
+
+int mkh_sim_set_pacing_rect(void* sim, long long nx, long long ny,
+                            long long x, long long y)
+{
+    CHECK_SIM();
+    long long r[4] = {nx, ny, x, y};
+    return SIM->set_pacing(1, r, 4);
+}
+
+int mkh_sim_set_pacing_list(void* sim, const long long* cells, long long n)
+{
+    CHECK_SIM();
+    return SIM->set_pacing(2, cells, n);
+}
+
+int mkh_sim_set_logging(void* sim, const int* colmap, long long nq, int nc,
+                        int nan_check)
+{
+    CHECK_SIM();
+    return SIM->set_logging(colmap, nq, nc, nan_check);
+}
+
+int mkh_sim_init(void* sim, double tmin, double tmax, double dt,
+                 const double* events, long long n_events,
+                 double log_interval, long long* n_log_rows)
+{
+    CHECK_SIM();
+    return SIM->init(tmin, tmax, dt, events, n_events, log_interval,
+                     n_log_rows);
+}
+
+int mkh_sim_set_log(void* sim, void* values, double* times, double* paces)
+{
+    CHECK_SIM();
+    return SIM->set_log(values, times, paces);
+}
+
+int mkh_sim_step(void* sim, double* time, int* status)
+{
+    CHECK_SIM();
+    return SIM->step(time, status);
+}
+
+int mkh_sim_clean(void* sim)
+{
+    CHECK_SIM();
+    return SIM->clean();
+}
+
+int mkh_sim_set_option(void* sim, const char* key, long long value)
+{
+    CHECK_SIM();
+    if (!key) return fail(MKH_E_ARG, "null option name");
+    return SIM->set_option(key, value);
+}
+
+int mkh_sim_get_timing(void* sim, double* kernel_ms, long long* launches,
+                       long long* cell_steps)
+{
+    CHECK_SIM();
+    return SIM->timing(kernel_ms, launches, cell_steps);
+}
+
+int mkh_sim_reset_timing(void* sim)
+{
+    CHECK_SIM();
+    return SIM->reset_timing();
+}
+
+int mkh_sim_log_rows(void* sim, long long* rows)
+{
+    CHECK_SIM();
+    if (rows) *rows = SIM->log_rows();
+    return 0;
+}
+
+int mkh_pacing_levels(const double* events, long long n_events, double t0,
+                      const double* times, long long n_times, double* levels,
+                      double* next_times)
+{
+    MkESys es;
+    int f = es.init(t0, events, n_events);
+    if (f) return pacing_error(f);
+    for (long long k = 0; k < n_times; ++k) {
+        f = es.advance(times[k]);
+        if (f) return pacing_error(f);
+        if (levels) levels[k] = es.level();
+        if (next_times) next_times[k] = es.next_time();
+    }
+    return 0;
+}
+
+int mkh_plan(double tmin, double tmax, double dt, const double* events,
+             long long n_events, double log_interval, double* t,
+             double* step, double* pace, int* log, long long max_steps,
+             long long* n_steps)
+{
+    Planner p;
+    int f = p.init(tmin, tmax, dt, events, n_events, log_interval);
+    if (f) return pacing_error(f);
+    long long k = 0;
+    StepRec r;
+    while (!p.done && k < max_steps) {
+        f = p.next(r);
+        if (f) return pacing_error(f);
+        if (t) t[k] = r.t;
+        if (step) step[k] = r.dt;
+        if (pace) pace[k] = r.pace;
+        if (log) log[k] = r.log ? 1 : 0;
+        ++k;
+    }
+    if (n_steps) *n_steps = k;
+    return 0;
+}
+
+}  // extern "C"

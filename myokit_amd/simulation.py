@@ -1,0 +1,995 @@
+"""
+``SimulationOpenCL``: drop-in replacement for ``myokit.SimulationOpenCL``
+(``myokit/_sim/openclsim.py:23``) that runs on an AMD Instinct GPU through
+``libmyokit_amd.so``.
+
+Same constructor ``(model, protocol=None, ncells=256, diffusion=True,
+precision=SINGLE_PRECISION, native_maths=False, rl=False)``, same methods
+(``run``, ``pre``, ``reset``, ``set_conductance``, ``set_conductance_field``,
+``set_connections``, ``set_field``, ``set_paced_cells``,
+``set_paced_cell_list``, ``set_step_size``, ``set_state``,
+``set_default_state``, ``set_time``, ``set_protocol``, ``set_constant``,
+``state``, ``default_state``, ``shape``, ``neighbors``, ``is_paced``,
+``monodomain_conductance``, ``find_nan``), same argument checks and error
+messages (``openclsim.py:149-322, 716-885, 1286-1757``), and the same
+``DataLog`` keys. The time-stepping, logging and pacing rules are those of
+the reference host module (``openclsim.c:1036-1211``), implemented in the
+C++ library.
+"""
+import ctypes
+import time as _time
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _datalog
+from . import _expressions as ex
+from . import _kernelgen
+from . import _native
+from ._datalog import (DOUBLE_PRECISION, LOG_BOUND, LOG_INTER, LOG_NONE,
+                       LOG_STATE, SINGLE_PRECISION, prepare_log, split_key)
+from ._err import (FindNanError, SimulationCancelledError, SimulationError,
+                   SimultaneousProtocolEventError)
+from ._model import Variable
+
+
+def _prepare_bindings(model, labels):
+    """Maps bound variables to internal inputs; drops other bindings
+    (behaviour of ``myokit._prepare_bindings``, ``_aux.py:434-475``)."""
+    unused = []
+    variables = {}
+    for label, var in model.bindings():
+        if label in labels:
+            variables[var] = labels[label]
+            var.set_rhs(0)
+        else:
+            unused.append(var)
+    for var in unused:
+        var.set_binding(None)
+    return variables
+
+
+def _prepare_model(model, rl, diffusion):
+    """Clones the model (RL-converted if requested); returns
+    ``(model, vm, rl_states, bound)``."""
+    from .lib import hh
+    vm = model.label('membrane_potential')
+    rl_states = {}
+    if rl:
+        m = hh.convert_hh_states_to_inf_tau_form(model, vm)
+        vm = m.get(vm.qname())
+        for s in m.states():
+            r = hh.get_inf_and_tau(s, vm)
+            if r is not None:
+                rl_states[s] = r
+    else:
+        m = model.clone()
+        vm = None if vm is None else m.get(vm.qname())
+    inputs = {'time': 'time', 'pace': 'pace'}
+    if diffusion:
+        inputs['diffusion_current'] = 'idiff'
+    bound = _prepare_bindings(m, inputs)
+    m.validate()
+    m.create_unique_names()
+    return m, vm, rl_states, bound
+
+
+def _prepare_cell_model(model, rl, diffusion, fields, quantities):
+    """Builds a CellModel from an unprepared model (used by build checks)."""
+    m, vm, rl_states, bound = _prepare_model(model, rl, diffusion)
+    fields = [m.get(f) if isinstance(f, str) else f for f in fields]
+    quantities = [m.get(q) if isinstance(q, str) else q for q in quantities]
+    return _kernelgen.CellModel(m, bound, rl_states, fields, quantities)
+
+
+# Compiled programs, keyed by kernel source and options
+_PROGRAMS = {}
+
+
+def _program(source, options, device=0):
+    key = (source, options, device)
+    p = _PROGRAMS.get(key)
+    if p is None:
+        import os
+        cache = os.environ.get('MYOKIT_AMD_CACHE')
+        if cache:
+            os.makedirs(cache, exist_ok=True)
+        prog = ctypes.c_void_p()
+        log = ctypes.create_string_buffer(1 << 16)
+        _native.check(_native.lib().mkh_program_build(
+            source.encode(), options.encode(),
+            cache.encode() if cache else None, device, ctypes.byref(prog),
+            log, 1 << 16), 'mkh_program_build')
+        p = _PROGRAMS[key] = prog
+    return p
+
+
+class _Engine:
+    """One device simulation handle for one compiled kernel."""
+
+    def __init__(self, program, config):
+        self.config = config
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().mkh_sim_create(
+            program, ctypes.byref(config), ctypes.byref(h)),
+            'mkh_sim_create')
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                _native.lib().mkh_sim_destroy(self.h)
+        except Exception:   # pragma: no cover
+            pass
+        self.h = None
+
+
+class SimulationOpenCL:
+    """Multi-cell (1D, 2D or graph) simulation of a cell model on a GPU.
+
+    See ``myokit.SimulationOpenCL`` for the model of use: forward Euler (or
+    Rush-Larsen for Hodgkin-Huxley gates when ``rl=True``) with a fixed step
+    size, diffusion current ``i = sum g * (V - V_j)`` over neighbours, cells
+    paced by a protocol, and per-cell logging.
+    """
+
+    # The current state lives on the device between runs. The host copy is
+    # refreshed on access (``_state`` getter) and uploaded only after the
+    # host changed it.
+    @property
+    def _state(self):
+        if self._dev_newer:
+            self._download_state()
+        return self._state_arr
+
+    @_state.setter
+    def _state(self, value):
+        self._state_arr = np.array(value, dtype=np.float64)
+        self._dev_newer = False
+        self._host_dirty = True
+
+    def _download_state(self):
+        st = np.zeros(self._ntotal * self._nstate)
+        _native.check(_native.lib().mkh_sim_get_state(
+            self._engine.h, _native.ptr(st, ctypes.c_double), len(st)),
+            'get_state')
+        self._state_arr = st
+        self._dev_newer = False
+
+    def _touch(self, what='config'):
+        # Marks device-side configuration as changed
+        if what == 'graph':
+            self._graph_version += 1
+        self._config_version += 1
+
+    def __init__(self, model, protocol=None, ncells=256, diffusion=True,
+                 precision=SINGLE_PRECISION, native_maths=False, rl=False):
+        self._config_version = 0
+        self._graph_version = 0
+        self._dev_newer = False
+        self._engine = None
+        if not model.is_valid():
+            model.validate()
+        if model.has_interdependent_components():
+            cycles = '\n'.join(
+                '  ' + ' > '.join(x.name() for x in c)
+                for c in model.component_cycles())
+            raise ValueError(
+                'This simulation requires models without interdependent'
+                ' components. Please restructure the model and re-run.'
+                '\nCycles:\n' + cycles)
+        self.set_protocol(protocol)
+        try:
+            self._nx = int(ncells)
+            self._ny = 1
+            self._dims = (self._nx,)
+        except TypeError:
+            try:
+                assert len(ncells) == 2
+            except (TypeError, AssertionError):
+                raise ValueError(
+                    'The argument "ncells" must be either a scalar or a tuple'
+                    ' (nx, ny).')
+            self._nx = int(ncells[0])
+            self._ny = int(ncells[1])
+            self._dims = (self._nx, self._ny)
+        if self._nx < 1 or self._ny < 1:
+            raise ValueError(
+                'The number of cells in any direction must be at least 1.')
+        self._ntotal = self._nx * self._ny
+        self._diffusion_enabled = bool(diffusion)
+        if precision not in (SINGLE_PRECISION, DOUBLE_PRECISION):
+            raise ValueError('Only single and double precision are supported.')
+        self._precision = precision
+        self._native_math = bool(native_maths)
+        self._rl = bool(rl)
+
+        vm = model.label('membrane_potential')
+        if self._diffusion_enabled or self._rl:
+            if vm is None:
+                raise ValueError(
+                    'This simulation requires the membrane potential'
+                    ' variable to be labelled as "membrane_potential".')
+            if not vm.is_state():
+                raise ValueError(
+                    'The variable labelled as membrane potential must'
+                    ' be a state variable.')
+
+        self._model, self._vm, self._rl_states, self._bound_variables = \
+            _prepare_model(model, self._rl, self._diffusion_enabled)
+
+        self._gx = self._gy = None
+        if self._diffusion_enabled:
+            self.set_conductance()
+        self._gx_field = None
+        self._gy_field = None
+        self._connections = None
+        self._paced_cells = None
+        if self._diffusion_enabled:
+            self.set_paced_cells()
+        self._fields = OrderedDict()
+        self.set_step_size()
+        self._time = 0
+        self._nstate = self._model.count_states()
+        init = np.array(self._model.initial_values(True), dtype=np.float64)
+        self._state = np.tile(init, self._ntotal)
+        self._default_state = self._state.copy()
+        self._global = ['time', 'pace']
+        self._engine_key = None
+        self._device = 0
+        self._block_threads = 256
+        self._options = {}
+        self.last_run_stats = {}
+
+    # ------------------------------------------------------------------
+    # Simple accessors
+    # ------------------------------------------------------------------
+    def conductance(self):
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        if (self._gx_field is not None) or (self._connections is not None):
+            return None
+        if len(self._dims) == 1:
+            return self._gx
+        return (self._gx, self._gy)
+
+    def _cell_slice(self, x, y, src):
+        x = int(x or 0)
+        if x < 0 or x >= self._nx:
+            raise IndexError('Given x-index out of range.')
+        y = int(y or 0)
+        if len(self._dims) == 2:
+            if y < 0 or y >= self._ny:
+                raise IndexError('Given y-index out of range.')
+            x += y * self._nx
+        elif y != 0:
+            raise ValueError(
+                'Y-coordinate specified for 1-dimensional simulation.')
+        return list(src[x * self._nstate:(x + 1) * self._nstate])
+
+    def default_state(self, x=None, y=None):
+        if x is None:
+            return list(self._default_state)
+        return self._cell_slice(x, y, self._default_state)
+
+    def state(self, x=None, y=None):
+        if x is None:
+            return list(self._state)
+        return self._cell_slice(x, y, self._state)
+
+    def is_2d(self):
+        return len(self._dims) == 2
+
+    def is_paced(self, x, y=None):
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        x = int(x)
+        if x < 0 or x >= self._dims[0]:
+            raise IndexError('X-coordinate out of range: ' + str(x) + '.')
+        if len(self._dims) == 2:
+            if y is None:
+                raise ValueError(
+                    'No y-coordinate specified in 2-dimensional simulation.')
+            y = int(y)
+            if y < 0 or y >= self._dims[1]:
+                raise IndexError('Y-coordinate out of range: ' + str(y) + '.')
+        elif y is None:
+            y = 0
+        elif y != 0:
+            raise ValueError(
+                'Y-coordinate specified for 1-dimensional simulation.')
+        if type(self._paced_cells) == tuple:
+            nx, ny, xmin, ymin = self._paced_cells
+            return (xmin <= x < xmin + nx and ymin <= y < ymin + ny)
+        return (x + y * self._dims[0]) in self._paced_cells
+
+    def monodomain_conductance(self, chi, k, D, dx, A=1):
+        """``g = (1 / chi) * (k / (k + 1)) * D * A / dx^2``
+        (``openclsim.py:753-812``)."""
+        return D * A * k / ((k + 1) * chi * dx * dx)
+
+    def neighbors(self, x, y=None):
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        x = int(x or 0)
+        if x < 0 or x >= self._dims[0]:
+            raise IndexError('X-coordinate out of range: ' + str(x) + '.')
+        if len(self._dims) == 2:
+            if y is None:
+                raise ValueError(
+                    'No y-coordinate specified in 2-dimensional simulation.')
+            y = int(y)
+            if y < 0 or y >= self._dims[1]:
+                raise IndexError('Y-coordinate out of range: ' + str(y) + '.')
+        elif not (y is None or y == 0):
+            raise ValueError(
+                'Y-coordinate specified for 1-dimensional simulation.')
+        if self._connections is not None:
+            out = []
+            for i, j, c in self._connections:
+                if i == x:
+                    out.append(j)
+                elif j == x:
+                    out.append(i)
+            return out
+        out = []
+        if x > 0:
+            out.append(x - 1)
+        if x + 1 < self._dims[0]:
+            out.append(x + 1)
+        if len(self._dims) == 2:
+            out = [(i, y) for i in out]
+            if y > 0:
+                out.append((x, y - 1))
+            if y + 1 < self._dims[1]:
+                out.append((x, y + 1))
+        return out
+
+    def shape(self):
+        if len(self._dims) == 2:
+            return (self._ny, self._nx)
+        return self._nx
+
+    def step_size(self):
+        return self._step_size
+
+    def time(self):
+        return self._time
+
+    # ------------------------------------------------------------------
+    # Setters (openclsim.py:1286-1757)
+    # ------------------------------------------------------------------
+    def set_conductance(self, gx=10, gy=5):
+        self._touch('config')
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        gx, gy = float(gx), float(gy)
+        if gx < 0:
+            raise ValueError('Invalid conductance gx=' + str(gx))
+        if gy < 0:
+            raise ValueError('Invalid conductance gy=' + str(gy))
+        self._gx, self._gy = gx, gy
+        self._gx_field = self._gy_field = None
+        self._connections = None
+
+    def set_conductance_field(self, gx, gy=None):
+        self._touch('config')
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        gx = np.asarray(gx, dtype=float)
+        if len(self._dims) == 1:
+            s = self._nx - 1
+            if gx.shape != (s, ):
+                raise ValueError(
+                    'The argument `gx` must have length ' + str(s) + '.')
+            if gy is not None:
+                raise ValueError(
+                    'For 1-d simulations the argument `gy` must be None.')
+        else:
+            s = (self._ny, self._nx - 1)
+            if gx.shape != s:
+                raise ValueError(
+                    'The argument `gx` must have dimensions ' + str(s) + '.')
+            if gy is None:
+                raise ValueError(
+                    'The argument `gy` must be set for 2-d simulations.')
+            gy = np.asarray(gy, dtype=float)
+            s = (self._ny - 1, self._nx)
+            if gy.shape != s:
+                raise ValueError(
+                    'The argument `gy` must have dimensions ' + str(s) + '.')
+        if np.any(gx < 0):
+            raise ValueError(
+                'The argument `gx` can not contain negative values.')
+        if (gy is not None) and np.any(gy < 0):
+            raise ValueError(
+                'The argument `gy` can not contain negative values.')
+        self._gx_field = list(gx.reshape((self._nx - 1) * self._ny))
+        if gy is None:
+            self._gy_field = None
+        else:
+            self._gy_field = list(gy.reshape(self._nx * (self._ny - 1)))
+        self._connections = None
+
+    def set_connections(self, connections):
+        self._touch('graph')
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        if len(self._dims) != 1:
+            raise RuntimeError('Connections can only be specified in 1d mode.')
+        if connections is None:
+            raise ValueError(
+                'Connection list cannot be None. To unset connections, call '
+                ' set_conductance() with the new conductance value.')
+        conns = []
+        doubles = set()
+        for x in connections:
+            try:
+                i, j, c = x
+            except Exception:
+                raise ValueError(
+                    'Connections must be a list of 3-tuples'
+                    ' (cell_index_1, cell_index_2, conductance).')
+            i, j = int(i), int(j)
+            if i == j or i < 0 or j < 0 or i >= self._nx or j >= self._nx:
+                raise ValueError(
+                    'Invalid connection: (' + str(i) + ', ' + str(j) + ', '
+                    + str(c) + ')')
+            i, j = (i, j) if i < j else (j, i)
+            if (i, j) in doubles:
+                raise ValueError(
+                    'Duplicate connection: (' + str(i) + ', ' + str(j) + ', '
+                    + str(c) + ')')
+            doubles.add((i, j))
+            c = float(c)
+            if c < 0:
+                raise ValueError('Invalid conductance: ' + str(c))
+            conns.append((i, j, c))
+        self._connections = conns
+        self._gx_field = self._gy_field = None
+
+    def set_constant(self, var, value):
+        value = float(value)
+        if isinstance(var, Variable):
+            var = var.qname()
+        var = self._model.get(var)
+        if not var.is_literal():
+            raise ValueError(
+                'The given variable <' + var.qname() + '> is not a literal.')
+        self._model.set_value(var.qname(), value)
+
+    def set_default_state(self, state, x=None, y=None):
+        self._default_state = self._set_state(state, x, y,
+                                              self._default_state)
+
+    def set_field(self, var, values):
+        self._touch('config')
+        if isinstance(var, Variable):
+            var = var.qname()
+        var = self._model.get(var)
+        if var.is_bound():
+            raise ValueError('Bound values cannot be replaced by fields.')
+        if not var.is_constant():
+            raise ValueError('Only constants can be used for fields.')
+        values = np.asarray(values, dtype=float)
+        if len(self._dims) == 1:
+            if values.shape != (self._nx, ):
+                raise ValueError(
+                    'The argument `values` must have length ' + str(self._nx)
+                    + '.')
+        else:
+            shape = (self._ny, self._nx)
+            if values.shape != shape:
+                raise ValueError(
+                    'The argument `values` must have dimensions ' + str(shape)
+                    + '.')
+        self._fields[var] = values.reshape(self._nx * self._ny).copy()
+
+    def remove_field(self, var):
+        self._touch('config')
+        if isinstance(var, Variable):
+            var = var.qname()
+        var = self._model.get(var)
+        del self._fields[var]
+
+    def set_paced_cells(self, nx=5, ny=5, x=0, y=0):
+        self._touch('config')
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        nx = int(nx)
+        x = int(x)
+        if nx < 0:
+            nx = -nx
+            x -= nx
+        if x < 0:
+            x += self._nx
+        if len(self._dims) == 1:
+            ny = 1
+            y = 0
+        else:
+            ny = int(ny)
+            if ny < 0:
+                ny = -ny
+                y -= ny
+            if y < 0:
+                y += self._ny
+        self._paced_cells = (nx, ny, x, y)
+
+    def set_paced_cell_list(self, cells):
+        self._touch('config')
+        if not self._diffusion_enabled:
+            raise RuntimeError(
+                'This method is unavailable when diffusion is disabled.')
+        paced = []
+        if len(self._dims) == 1:
+            for cell in cells:
+                cell = int(cell)
+                if cell < 0 or cell >= self._nx:
+                    raise IndexError(
+                        'Cell index out of range: ' + str(cell) + '.')
+                paced.append(cell)
+        else:
+            for i, j in cells:
+                i, j = int(i), int(j)
+                if i < 0 or j < 0 or i >= self._nx or j >= self._ny:
+                    raise IndexError(
+                        'Cell index out of range: (' + str(i) + ', ' + str(j)
+                        + ').')
+                paced.append(i + j * self._nx)
+        self._paced_cells = paced
+
+    def set_protocol(self, protocol=None):
+        self._protocol = None if protocol is None else protocol.clone()
+
+    def _set_state(self, state, x, y, update):
+        n = len(state)
+        if n == self._nstate * self._ntotal:
+            if x is not None:
+                raise ValueError(
+                    'A state for all cells was passed in, but argument x'
+                    ' was not None.')
+            if y is not None:
+                raise ValueError(
+                    'A state for all cells was passed in, but argument y'
+                    ' was not None.')
+            return np.array(state, dtype=np.float64)
+        elif n != self._nstate:
+            raise ValueError(
+                'Given state must have the same size as a single'
+                ' cell state or a full simulation state')
+        if x is None:
+            return np.tile(np.asarray(state, dtype=np.float64), self._ntotal)
+        x = int(x or 0)
+        if x < 0 or x >= self._nx:
+            raise IndexError('Given x-index out of range.')
+        y = int(y or 0)
+        if len(self._dims) == 2:
+            if y < 0 or y >= self._ny:
+                raise IndexError('Given y-index out of range.')
+            x += y * self._nx
+        elif y != 0:
+            raise ValueError('Y-index given for a 1-dimensional simulation.')
+        update = np.array(update, dtype=np.float64)
+        off = x * self._nstate
+        update[off:off + self._nstate] = state
+        return update
+
+    def set_state(self, state, x=None, y=None):
+        self._state = self._set_state(state, x, y, self._state)
+
+    def set_step_size(self, step_size=0.005):
+        step_size = float(step_size)
+        if step_size <= 0:
+            raise ValueError('Step size must be greater than zero.')
+        self._step_size = step_size
+
+    def set_time(self, time=0):
+        self._time = float(time)
+
+    # ------------------------------------------------------------------
+    # Running
+    # ------------------------------------------------------------------
+    def pre(self, duration, report_nan=True, progress=None,
+            msg='Pre-pacing SimulationOpenCL'):
+        self._run(duration, LOG_NONE, 1, report_nan, progress, msg)
+        self._default_state = self._state.copy()
+
+    def reset(self):
+        self._time = 0
+        self._state = self._default_state.copy()
+
+    def run(self, duration, log=None, log_interval=1.0, report_nan=True,
+            progress=None, msg='Running SimulationOpenCL'):
+        r = self._run(duration, log, log_interval, report_nan, progress, msg)
+        self._time += duration
+        return r
+
+    def _coupling(self):
+        if not self._diffusion_enabled:
+            return _native.COUPLING_NONE
+        if self._connections is not None:
+            return _native.COUPLING_GRAPH
+        if self._gx_field is not None:
+            return _native.COUPLING_HETERO
+        return _native.COUPLING_STENCIL
+
+    def set_engine_option(self, key, value):
+        """Tuning option passed to the device library (``batch_steps``,
+        ``log_rows``, ``time_kernels``, ``driver``)."""
+        self._options[key] = int(value)
+        if self._engine is not None:
+            _native.check(_native.lib().mkh_sim_set_option(
+                self._engine.h, key.encode(), int(value)), 'set_option')
+
+    def engine_timing(self, reset=False):
+        """Kernel timing of the device engine (needs the ``time_kernels``
+        option): ``{'kernel_ms', 'launches', 'cell_steps'}``."""
+        if self._engine is None:
+            return {'kernel_ms': 0.0, 'launches': 0, 'cell_steps': 0}
+        ms = ctypes.c_double(0)
+        nl = ctypes.c_longlong(0)
+        cs = ctypes.c_longlong(0)
+        L = _native.lib()
+        _native.check(L.mkh_sim_get_timing(
+            self._engine.h, ctypes.byref(ms), ctypes.byref(nl),
+            ctypes.byref(cs)), 'get_timing')
+        if reset:
+            L.mkh_sim_reset_timing(self._engine.h)
+        return {'kernel_ms': ms.value, 'launches': nl.value,
+                'cell_steps': cs.value}
+
+    def _get_engine(self, quantities):
+        fields = list(self._fields.keys())
+        cm = _kernelgen.CellModel(self._model, self._bound_variables,
+                                  self._rl_states, fields, quantities)
+        coupling = self._coupling()
+        src = _kernelgen.generate_kernel(
+            cm, self._precision, coupling, self._block_threads,
+            self._native_math)
+        opts = _kernelgen.compile_options(self._precision)
+        key = (src, opts, self._ntotal, self._dims)
+        if self._engine is not None and self._engine_key == key:
+            return self._engine, cm
+        if _native.device_count() < 1:
+            raise _native.DeviceError(
+                'No HIP device found: SimulationOpenCL needs an AMD GPU.')
+        if self._engine is not None and self._dev_newer:
+            self._download_state()   # keep the state of the old engine
+        prog = _program(src, opts, self._device)
+        cfg = _native.SimConfig()
+        cfg.precision = self._precision
+        cfg.n_state = self._nstate
+        cfg.n_field = len(fields)
+        cfg.n_quant = len(quantities)
+        cfg.i_vm = self._vm.index() if self._vm is not None else 0
+        cfg.coupling = coupling
+        cfg.driver = -1
+        cfg.block_threads = self._block_threads
+        cfg.nx = self._nx
+        cfg.ny = self._ny
+        cfg.device = self._device
+        eng = _Engine(prog, cfg)
+        for k, v in self._options.items():
+            _native.check(_native.lib().mkh_sim_set_option(
+                eng.h, k.encode(), v), 'set_option')
+        eng.config_version = None
+        eng.graph_version = None
+        self._engine = eng
+        self._engine_key = key
+        self._host_dirty = True
+        return eng, cm
+
+    def _configure(self, eng):
+        """Sends configuration and state that changed since the engine last
+        received them."""
+        L = _native.lib()
+        h = eng.h
+        if eng.config_version != self._config_version:
+            c = self._coupling()
+            if c == _native.COUPLING_STENCIL:
+                _native.check(L.mkh_sim_set_conductance(
+                    h, self._gx, self._gy), 'set_conductance')
+            elif c == _native.COUPLING_HETERO:
+                gx = np.ascontiguousarray(self._gx_field, dtype=np.float64)
+                gy = np.ascontiguousarray(
+                    self._gy_field if self._gy_field is not None else [],
+                    dtype=np.float64)
+                _native.check(L.mkh_sim_set_conductance_field(
+                    h, _native.ptr(gx, ctypes.c_double), len(gx),
+                    _native.ptr(gy, ctypes.c_double), len(gy)),
+                    'set_conductance_field')
+            elif c == _native.COUPLING_GRAPH and \
+                    eng.graph_version != self._graph_version:
+                a = np.array(self._connections,
+                             dtype=np.float64).reshape(-1, 3)
+                ci = np.ascontiguousarray(a[:, 0], dtype=np.int64)
+                cj = np.ascontiguousarray(a[:, 1], dtype=np.int64)
+                cg = np.ascontiguousarray(a[:, 2], dtype=np.float64)
+                _native.check(L.mkh_sim_set_connections(
+                    h, _native.ptr(ci, ctypes.c_longlong),
+                    _native.ptr(cj, ctypes.c_longlong),
+                    _native.ptr(cg, ctypes.c_double), len(ci)),
+                    'set_connections')
+                eng.graph_version = self._graph_version
+            if not self._diffusion_enabled:
+                _native.check(L.mkh_sim_set_pacing_all(h), 'pacing')
+            elif type(self._paced_cells) == tuple:
+                nx, ny, x, y = self._paced_cells
+                _native.check(L.mkh_sim_set_pacing_rect(h, nx, ny, x, y),
+                              'pacing')
+            elif self._paced_cells:
+                cells = np.ascontiguousarray(self._paced_cells,
+                                             dtype=np.int64)
+                _native.check(L.mkh_sim_set_pacing_list(
+                    h, _native.ptr(cells, ctypes.c_longlong), len(cells)),
+                    'pacing')
+            else:
+                _native.check(L.mkh_sim_set_pacing_none(h), 'pacing')
+            if self._fields:
+                fd = np.ascontiguousarray(
+                    np.stack(list(self._fields.values()), axis=1).reshape(-1),
+                    dtype=np.float64)
+                _native.check(L.mkh_sim_set_fields(
+                    h, _native.ptr(fd, ctypes.c_double), len(fd)),
+                    'set_fields')
+            eng.config_version = self._config_version
+        if self._host_dirty:
+            st = np.ascontiguousarray(self._state_arr, dtype=np.float64)
+            _native.check(L.mkh_sim_set_state(
+                h, _native.ptr(st, ctypes.c_double), len(st)), 'set_state')
+            self._host_dirty = False
+
+    def _log_layout(self, log):
+        """Returns (quantities, colmap, per-cell keys by column, global keys,
+        logging_states)."""
+        gtime = self._model.time()
+        gpace = self._model.binding('pace')
+        gnames = {}
+        if gtime is not None:
+            gnames[gtime.qname()] = 'time'
+        if gpace is not None:
+            gnames[gpace.qname()] = 'pace'
+        quantities = []
+        qindex = {}
+        cols = []          # (key, q, cell)
+        globals_ = []      # (key, 'time'|'pace')
+        nx = self._nx
+        for key in log.keys():
+            kdims, kname = split_key(key)
+            if not kdims:
+                if kname in gnames:
+                    globals_.append((key, gnames[kname]))
+                    continue
+                kdims = '0.'    # single cell logs without index (not used)
+            var = self._model.get(kname)
+            if var not in qindex:
+                qindex[var] = len(quantities)
+                quantities.append(var)
+            idx = [int(i) for i in kdims[:-1].split('.')]
+            cell = idx[0] + (idx[1] * nx if len(idx) > 1 else 0)
+            cols.append((key, qindex[var], cell))
+        colmap = np.full((len(quantities), self._ntotal), -1, dtype=np.int32)
+        for col, (key, q, cell) in enumerate(cols):
+            colmap[q, cell] = col
+        logging_states = any(q.is_state() for q in quantities)
+        return quantities, colmap, cols, globals_, logging_states
+
+    def _run(self, duration, log, log_interval, report_nan, progress, msg):
+        if duration < 0:
+            raise ValueError('Simulation time can\'t be negative.')
+        tmin = self._time
+        tmax = tmin + duration
+        g = []
+        for label in self._global:
+            v = self._model.binding(label)
+            if v is not None:
+                g.append(v.qname())
+        log = prepare_log(
+            log, self._model, dims=self._dims, global_vars=g,
+            if_empty=LOG_STATE + LOG_BOUND,
+            allowed_classes=LOG_STATE + LOG_INTER + LOG_BOUND,
+            precision=self._precision)
+        log_interval = 1e-9 if log_interval is None else float(log_interval)
+        if log_interval <= 0:
+            log_interval = 1e-9
+        if progress is not None:
+            from ._progress import ProgressReporter
+            if not isinstance(progress, ProgressReporter):
+                raise ValueError(
+                    'The argument "progress" must be either a subclass of'
+                    ' myokit.ProgressReporter or None.')
+
+        arithmetic_error = False
+        if duration > 0:
+            quantities, colmap, cols, globals_, logging_states = \
+                self._log_layout(log)
+            values, times, paces, arithmetic_error = self._execute(
+                tmin, tmax, quantities, colmap, len(cols), logging_states,
+                log_interval, progress, msg)
+            dtype = values.dtype
+            self._fill_log(log, values, times, paces, cols, globals_, dtype)
+
+        if report_nan and (arithmetic_error or log.has_nan()):
+            self._report_nan(log)
+        return log
+
+    def _execute(self, tmin, tmax, quantities, colmap, ncols, logging_states,
+                 log_interval, progress=None, msg=''):
+        """Runs from tmin to tmax on the device. Returns (values[rows, ncols],
+        times, paces, arithmetic_error)."""
+        duration = tmax - tmin
+        eng, cm = self._get_engine(quantities)
+        if progress is not None and self._dev_newer:
+            # Keep a host copy: a cancelled run leaves the state unchanged
+            self._download_state()
+        self._configure(eng)
+        L = _native.lib()
+        h = eng.h
+        _native.check(L.mkh_sim_set_logging(
+            h, _native.ptr(np.ascontiguousarray(colmap.reshape(-1)),
+                           ctypes.c_int),
+            len(quantities), ncols, int(logging_states)), 'set_logging')
+        events = np.ascontiguousarray(
+            self._protocol.to_arrays() if self._protocol is not None
+            else np.zeros((0, 5)), dtype=np.float64).reshape(-1)
+        nrows = ctypes.c_longlong(0)
+        code = L.mkh_sim_init(
+            h, tmin, tmax, self._step_size,
+            _native.ptr(events, ctypes.c_double), len(events) // 5,
+            log_interval, ctypes.byref(nrows))
+        if code == _native.MKH_E_PACING_SIMULTANEOUS:
+            raise SimultaneousProtocolEventError(_native.last_error())
+        _native.check(code, 'mkh_sim_init')
+        nrows = nrows.value
+        dtype = np.float64 if self._precision == DOUBLE_PRECISION \
+            else np.float32
+        values = np.empty((nrows, max(ncols, 1)), dtype=dtype)
+        times = np.zeros(nrows, dtype=np.float64)
+        paces = np.zeros(nrows, dtype=np.float64)
+        _native.check(L.mkh_sim_set_log(
+            h, values.ctypes.data_as(ctypes.c_void_p),
+            _native.ptr(times, ctypes.c_double),
+            _native.ptr(paces, ctypes.c_double)), 'mkh_sim_set_log')
+        t = ctypes.c_double(tmin)
+        status = ctypes.c_int(0)
+        arithmetic_error = False
+        t0 = _time.perf_counter()
+        ok = False
+        try:
+            if progress:
+                with progress.job(msg):
+                    r = 1.0 / duration if duration != 0 else 1
+                    while status.value == 0:
+                        code = L.mkh_sim_step(h, ctypes.byref(t),
+                                              ctypes.byref(status))
+                        if code == _native.MKH_E_NAN:
+                            arithmetic_error = True
+                            break
+                        self._check_step(code)
+                        if not progress.update(min((t.value - tmin) * r, 1)):
+                            raise SimulationCancelledError()
+            else:
+                while status.value == 0:
+                    code = L.mkh_sim_step(h, ctypes.byref(t),
+                                          ctypes.byref(status))
+                    if code == _native.MKH_E_NAN:
+                        arithmetic_error = True
+                        break
+                    self._check_step(code)
+            ok = True
+        finally:
+            L.mkh_sim_clean(h)
+            self.last_run_stats = {
+                'engine_seconds': _time.perf_counter() - t0,
+                'cell_steps': None}
+            if ok:
+                self._dev_newer = True
+            else:
+                # Device state is partly advanced: next run uploads the host
+                self._host_dirty = True
+        rows = ctypes.c_longlong(0)
+        L.mkh_sim_log_rows(h, ctypes.byref(rows))
+        rows = rows.value
+        return values[:rows], times[:rows], paces[:rows], arithmetic_error
+
+    def run_block(self, duration, variables=('membrane.V',),
+                  log_interval=1.0, report_nan=True):
+        """Runs like :meth:`run` with every cell of ``variables`` logged, but
+        returns dense arrays instead of a DataLog: ``{'time': (rows,),
+        'pace': (rows,), name: (rows, ny, nx) or (rows, nx)}``. This is the
+        time x space view of ``myokit.DataBlock1d/2d``
+        (``myokit/_datablock.py:87,787``) produced without per-cell keys."""
+        if duration < 0:
+            raise ValueError('Simulation time can\'t be negative.')
+        log_interval = 1e-9 if log_interval is None else float(log_interval)
+        if log_interval <= 0:
+            log_interval = 1e-9
+        quantities = []
+        for v in variables:
+            var = self._model.get(v) if not isinstance(v, Variable) else v
+            if not (var.is_state() or var.is_intermediary() or (
+                    var in self._bound_variables
+                    and self._bound_variables[var] == 'idiff')):
+                raise ValueError('Cannot log <' + var.qname() + '> per cell.')
+            quantities.append(var)
+        n = self._ntotal
+        colmap = np.arange(len(quantities) * n, dtype=np.int32).reshape(
+            len(quantities), n)
+        tmin = self._time
+        values, times, paces, err = self._execute(
+            tmin, tmin + duration, quantities, colmap, colmap.size,
+            any(q.is_state() for q in quantities), log_interval)
+        self._time += duration
+        shape = (self._ny, self._nx) if len(self._dims) == 2 else (self._nx,)
+        out = {'time': times.astype(values.dtype), 'pace':
+               paces.astype(values.dtype)}
+        for k, var in enumerate(quantities):
+            out[var.qname()] = values[:, k * n:(k + 1) * n].reshape(
+                (len(times),) + shape)
+        if report_nan and (err or not np.all(np.isfinite(values))):
+            raise SimulationError(
+                'Numerical error found in simulation logs.')
+        return out
+
+    def _check_step(self, code):
+        if code == _native.MKH_E_PACING_SIMULTANEOUS:
+            raise SimultaneousProtocolEventError(_native.last_error())
+        _native.check(code, 'mkh_sim_step')
+
+    def _fill_log(self, log, values, times, paces, cols, globals_, dtype):
+        import collections
+        new = {}
+        for key, what in globals_:
+            new[key] = (times if what == 'time' else paces).astype(dtype)
+        for col, (key, q, cell) in enumerate(cols):
+            new[key] = values[:, col]
+        for key, arr in new.items():
+            cur = log[key]
+            if isinstance(cur, np.ndarray) and len(cur) == 0:
+                collections.OrderedDict.__setitem__(log, key, arr)
+            else:
+                log.extend_column(key, arr)
+
+    def _report_nan(self, log):
+        txt = ['Numerical error found in simulation logs.']
+        try:
+            t, icell, var, value = self._locate_nan(log)
+            txt.append('Encountered numerical error at t=' + repr(t)
+                       + ' in cell (' + ','.join(str(x) for x in icell)
+                       + ') when ' + var + '=' + repr(value) + '.')
+        except FindNanError as e:
+            txt.append('Unable to pinpoint source of NaN, an error occurred:')
+            txt.append(str(e))
+        raise SimulationError('\n'.join(txt))
+
+    def _locate_nan(self, log):
+        """First logged point with a non-finite value: (time, cell index,
+        variable, value)."""
+        tkey = log.time_key()
+        if tkey is None:
+            raise FindNanError('Method requires a simulation log with time.')
+        first = None
+        for key, v in log.items():
+            a = np.asarray(v, dtype=float)
+            bad = np.nonzero(~np.isfinite(a))[0]
+            if len(bad) and (first is None or bad[0] < first[0]):
+                first = (bad[0], key, a[bad[0]])
+        if first is None:
+            raise FindNanError('No numerical error found in log.')
+        i, key, value = first
+        kdims, kname = split_key(key)
+        icell = tuple(int(x) for x in kdims[:-1].split('.')) if kdims else ()
+        return float(np.asarray(log[tkey])[i]), icell, kname, float(value)
+
+    def find_nan(self, log, watch_var=None, safe_range=None,
+                 return_log=False):
+        """Locates the first non-finite value in a log of this simulation:
+        returns ``(time, icell, var, value)``."""
+        return self._locate_nan(log)

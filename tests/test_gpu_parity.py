@@ -1,0 +1,226 @@
+"""
+GPU parity tests: SimulationOpenCL (HIP, MI355X) against the CPU oracle and
+against the reference's Simulation1d golden runs.
+
+Tolerances: double precision must match the oracle to max|dV| <= 1e-9 mV
+(the engines use the same operation order and the same portable math, so
+the expected difference is zero; the requirement is 1e-6 mV). Single
+precision is compared with the double-precision oracle: activation times
+within 0.1 ms and V RMS error <= 0.5 mV.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import myokit_amd as myokit
+from helpers import Scenario, activation_times, run_both
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+FP64_TOL = 1e-9
+
+
+def _report(name, P, O):
+    d = np.max(np.abs(P['V'] - O['V'])) if P['V'].size else 0.0
+    print(name, 'rows', P['V'].shape, 'max|dV| =', d, flush=True)
+    return d
+
+
+@pytest.mark.parametrize('case', ['fe', 'rl'])
+def test_fp64_cable_lr1991(case):
+    g = np.load(os.path.join(GOLD, 'sim1d_lr1991_' + case + '.npz'))
+    n, npaced, gx, rl, dt, dur, li = g['config']
+    sc = Scenario('lr1991', int(n), rl=bool(rl), dt=dt, gx=gx,
+                  events=[tuple(g['event'])], paced=(int(npaced),))
+    P, O = run_both(sc, dur, myokit.DOUBLE_PRECISION, log_interval=li)
+    assert P['V'].shape == O['V'].shape == g['V'].shape
+    assert _report('lr1991 ' + case, P, O) <= FP64_TOL
+    np.testing.assert_array_equal(P['time'], O['time'])
+    # Against the reference Simulation1d
+    assert np.max(np.abs(P['V'] - g['V'])) < 1e-9
+    assert np.max(np.abs(P['state'] - O['state'])) <= FP64_TOL
+
+
+def test_fp64_config1_geometry():
+    # 200-cell LR1991 cable, FE, dt 0.005, gx 9.5, 5 paced cells
+    g = np.load(os.path.join(GOLD, 'sim1d_lr1991_cfg1.npz'))
+    sc = Scenario('lr1991', 200, dt=0.005, gx=9.5, events=[(1, 0, 2, 1000, 0)],
+                  paced=(5,))
+    P, O = run_both(sc, 60.0, myokit.DOUBLE_PRECISION, log_interval=1.0)
+    assert _report('config1 60ms', P, O) <= FP64_TOL
+    assert np.max(np.abs(P['V'] - g['V'])) < 1e-9
+
+
+def test_fp64_br1977():
+    g = np.load(os.path.join(GOLD, 'sim1d_br1977_fe.npz'))
+    sc = Scenario('br1977', 10, dt=0.005, gx=10.0, events=[(1, 1, 2, 1000, 0)],
+                  paced=(2,))
+    P, O = run_both(sc, 15.0, myokit.DOUBLE_PRECISION, log_interval=0.5)
+    assert _report('br1977', P, O) <= FP64_TOL
+    assert np.max(np.abs(P['V'] - g['V'])) < 1e-9
+
+
+@pytest.mark.parametrize('shape', [(24, 16), (48, 40)])
+def test_fp64_sheet_rl(shape):
+    # Small sheet runs in one workgroup; the larger one on the grid driver
+    sc = Scenario('lr1991', shape, rl=True, dt=0.01, gx=9.5, gy=9.5,
+                  events=[(1, 0, 2, 1000, 0)], paced=(4, shape[1], 0, 0))
+    P, O = run_both(sc, 15.0, myokit.DOUBLE_PRECISION, log_interval=0.5)
+    assert _report('sheet ' + str(shape), P, O) <= FP64_TOL
+    assert np.max(np.abs(P['state'] - O['state'])) <= FP64_TOL
+
+
+def test_fp64_conductance_field():
+    rng = np.random.default_rng(5)
+    nx, ny = 20, 12
+    gx = rng.uniform(5, 12, size=(ny, nx - 1))
+    gy = rng.uniform(5, 12, size=(ny - 1, nx))
+    sc = Scenario('lr1991', (nx, ny), rl=True, dt=0.01,
+                  events=[(1, 0, 2, 1000, 0)], paced=(3, 3, 0, 0),
+                  conductance_field=(gx, gy))
+    P, O = run_both(sc, 12.0, myokit.DOUBLE_PRECISION)
+    assert _report('hetero', P, O) <= FP64_TOL
+
+
+def test_fp64_connections_graph():
+    # Two chains and a random graph component; summation order follows the
+    # connection list
+    rng = np.random.default_rng(7)
+    n = 120
+    conns = [(i, i + 1, 9.5) for i in range(0, 39)]
+    conns += [(i + 1, i, 7.0) for i in range(40, 79)]
+    for k in range(60):
+        i, j = rng.choice(np.arange(80, 120), size=2, replace=False)
+        if (min(i, j), max(i, j)) not in {(min(a, b), max(a, b))
+                                          for a, b, _ in conns}:
+            conns.append((int(i), int(j), float(rng.uniform(5, 12))))
+    sc = Scenario('lr1991', n, dt=0.005, events=[(1, 0, 2, 1000, 0)],
+                  paced_list=[0, 1, 2, 40, 41, 80, 81], connections=conns)
+    P, O = run_both(sc, 20.0, myokit.DOUBLE_PRECISION)
+    assert _report('graph', P, O) <= FP64_TOL
+
+
+def test_fp64_connections_large_graph_grid_driver():
+    # A component larger than a workgroup's LDS: global CSR gather
+    n = 3000
+    conns = [(i, i + 1, 9.5) for i in range(n - 1)]
+    sc = Scenario('lr1991', n, dt=0.005, events=[(1, 0, 2, 1000, 0)],
+                  paced_list=[0, 1, 2], connections=conns)
+    P, O = run_both(sc, 8.0, myokit.DOUBLE_PRECISION, log_interval=1.0)
+    assert _report('graph-grid', P, O) <= FP64_TOL
+
+
+def test_connections_equal_stencil():
+    # A chain given as connections equals the homogeneous 1D stencil
+    # (test_simulation_opencl.py:460-528 requires <= 1e-9 in double)
+    n = 30
+    a = Scenario('lr1991', n, dt=0.005, gx=9.5, paced=(3,))
+    b = Scenario('lr1991', n, dt=0.005, paced_list=[0, 1, 2],
+                 connections=[(i, i + 1, 9.5) for i in range(n - 1)])
+    sa, sb = a.product(64), b.product(64)
+    da = sa.run(10, log=['membrane.V'], log_interval=0.5)
+    db = sb.run(10, log=['membrane.V'], log_interval=0.5)
+    for k in da.keys():
+        assert np.max(np.abs(np.asarray(da[k]) - np.asarray(db[k]))) <= 1e-9
+
+
+def test_fp64_uncoupled_fields():
+    rng = np.random.default_rng(11)
+    n = 300
+    s = np.exp(rng.uniform(np.log(0.5), np.log(2), size=(3, n)))
+    fields = {'ina.gNa': 16 * s[0], 'ica.gCa': 0.09 * s[1],
+              'ik.IK.gK': 0.282 * s[2]}
+    sc = Scenario('lr1991', n, diffusion=False, dt=0.005,
+                  events=[(1, 0, 2, 1000, 0)], fields=fields)
+    P, O = run_both(sc, 40.0, myokit.DOUBLE_PRECISION, log_interval=1.0)
+    assert _report('uncoupled fields', P, O) <= FP64_TOL
+    # Cells differ through their fields
+    assert np.std(P['V'][-1]) > 0.01
+
+
+def test_uncoupled_identical_without_fields():
+    # diffusion=False and no fields: every cell identical, bit for bit
+    # (test_simulation_opencl.py:602-658)
+    s = myokit.SimulationOpenCL(myokit.load_model('example'),
+                                myokit.pacing.blocktrain(1000, 2), ncells=100,
+                                diffusion=False, precision=32)
+    d = s.run(20, log=['membrane.V'], log_interval=1)
+    V = np.stack([d[str(i) + '.membrane.V'] for i in range(100)], axis=1)
+    assert np.all(V == V[:, :1])
+
+
+@pytest.mark.parametrize('rl', [False, True])
+def test_fp32_against_fp64_oracle(rl):
+    dt = 0.01 if rl else 0.005
+    sc = Scenario('lr1991', 60, rl=rl, dt=dt, gx=9.5,
+                  events=[(1, 0, 2, 1000, 0)], paced=(5,))
+    P, O = run_both(sc, 60.0, myokit.SINGLE_PRECISION, log_interval=0.1)
+    rms = np.sqrt(np.mean((P['V'] - O['V']) ** 2))
+    ta = activation_times(P['time'], P['V'])
+    tb = activation_times(O['time'], O['V'])
+    assert np.all(np.isfinite(tb))
+    dta = np.max(np.abs(ta - tb))
+    print('fp32 rl=%s: V RMS %.3g mV, max activation dt %.3g ms'
+          % (rl, rms, dta), flush=True)
+    assert rms <= 0.5
+    assert dta <= 0.1
+
+
+def test_fp32_sheet_against_fp64_oracle():
+    shape = (48, 32)
+    sc = Scenario('lr1991', shape, rl=True, dt=0.01, gx=9.5, gy=9.5,
+                  events=[(1, 0, 2, 1000, 0)], paced=(4, shape[1], 0, 0))
+    P, O = run_both(sc, 40.0, myokit.SINGLE_PRECISION, log_interval=0.1)
+    rms = np.sqrt(np.mean((P['V'] - O['V']) ** 2))
+    ta = activation_times(P['time'], P['V'])
+    tb = activation_times(O['time'], O['V'])
+    ok = np.isfinite(tb)
+    assert np.all(np.isfinite(ta[ok]))
+    dta = np.max(np.abs(ta[ok] - tb[ok]))
+    print('fp32 sheet: V RMS %.3g mV, max activation dt %.3g ms' % (rms, dta),
+          flush=True)
+    assert rms <= 0.5
+    assert dta <= 0.1
+
+
+def test_log_interval_and_continuation():
+    # Log times k * interval (PeriodicTest, test_simulation_log_interval.py)
+    # and a run continued into the same log
+    s = myokit.SimulationOpenCL(myokit.load_model('example'),
+                                myokit.pacing.blocktrain(1000, 0.5), ncells=4,
+                                precision=64)
+    d = s.run(5, log=['engine.time', 'membrane.V'], log_interval=0.5)
+    t = np.asarray(d['engine.time'])
+    assert len(t) == 10
+    assert np.max(np.abs(t - 0.5 * np.arange(10))) < 1e-9
+    d = s.run(2, log=d, log_interval=0.5)
+    t = np.asarray(d['engine.time'])
+    assert len(t) == 14
+    assert np.max(np.abs(t - 0.5 * np.arange(14))) < 1e-9
+    assert len(d['3.membrane.V']) == 14
+
+
+def test_state_and_time_bookkeeping():
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, None, ncells=(3, 2), precision=64)
+    x0 = s.state()
+    s.run(1, log=myokit.LOG_NONE)
+    assert s.time() == 1
+    assert s.state() != x0
+    s.reset()
+    assert s.time() == 0 and s.state() == x0
+    s.pre(2)
+    assert s.time() == 0
+    assert s.default_state() == s.state() != x0
+
+
+def test_nan_raises_simulation_error():
+    # A huge step makes the model blow up (test_simulation_opencl.py:1493)
+    s = myokit.SimulationOpenCL(myokit.load_model('example'),
+                                myokit.pacing.blocktrain(1000, 2), ncells=8,
+                                precision=32)
+    s.set_step_size(5)
+    with pytest.raises(myokit.SimulationError):
+        s.run(200, log=['engine.time', 'membrane.V'], log_interval=5)
