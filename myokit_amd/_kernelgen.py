@@ -63,7 +63,7 @@ def real_literal(value, precision):
 # Function names per precision: (double, float, float-native)
 _FUNCS = {
     'sqrt': ('__builtin_sqrt', 'sqrtf', 'sqrtf'),
-    'exp': ('mk_exp', 'expf', '__expf'),
+    'exp': ('mk_exp', 'mk_expf', 'mk_expf_native'),
     'log': ('mk_log', 'logf', '__logf'),
     'log10': ('mk_log10', 'log10f', '__log10f'),
     'sin': ('mk_sin', 'sinf', '__sinf'),
@@ -140,7 +140,19 @@ class HipExpressionWriter:
         return self._infix(e, '*')
 
     def _ex_Divide(self, e):
-        return self._infix(e, '/')
+        if self.precision == DOUBLE_PRECISION:
+            return self._infix(e, '/')
+        # Single precision: a * rcp(b) (v_rcp_f32, within the 2.5 ulp that
+        # OpenCL allows for single-precision division); a literal divisor
+        # becomes a multiplication by its reciprocal.
+        a, b = e
+        if isinstance(b, ex.Number):
+            return ('(' + self.ex(a) + ' * '
+                    + real_literal(1.0 / b.eval(), self.precision) + ')')
+        if isinstance(a, ex.Number) and a.eval() == 1.0:
+            return '__builtin_amdgcn_rcpf(' + self.ex(b) + ')'
+        return ('(' + self.ex(a) + ' * __builtin_amdgcn_rcpf(' + self.ex(b)
+                + '))')
 
     def _ex_Quotient(self, e):
         # Rounds towards minus infinity, as myokit.Quotient
@@ -168,7 +180,65 @@ class HipExpressionWriter:
         return self._func('sqrt', e)
 
     def _ex_Exp(self, e):
+        if self.precision == SINGLE_PRECISION:
+            aff = self.affine(e[0])
+            if aff is not None:
+                # exp(k*x + m) = 2^(x * k*log2(e) + m*log2(e)): one fma and
+                # one v_exp_f32. The rounding error of the fused argument is
+                # no larger than that of evaluating k*x + m in single
+                # precision, which the reference's float kernel also has.
+                x, k, m = aff
+                l2e = 1.4426950408889634
+                return ('__builtin_amdgcn_exp2f(__builtin_fmaf(' + x + ', '
+                        + real_literal(k * l2e, self.precision) + ', '
+                        + real_literal(m * l2e, self.precision) + '))')
         return self._func('exp', e)
+
+    def affine(self, e):
+        """If ``e`` is ``k * x + m`` for one name ``x`` (not a host
+        constant) and literal ``k``, ``m``, returns ``(c_name_of_x, k, m)``
+        with k, m in double precision; else None."""
+        r = self._affine(e)
+        if r is None or r[0] is None or r[1] == 0:
+            return None
+        return r
+
+    def _affine(self, e):
+        # Returns (name_text or None, k, m)
+        if isinstance(e, ex.Number):
+            return (None, 0.0, e.eval())
+        if isinstance(e, ex.Name):
+            var = e.var()
+            if e not in self.names and var in self.constants:
+                return (None, 0.0, float(self.constants[var]))
+            return (self.ex(e), 1.0, 0.0)
+        if isinstance(e, ex.PrefixPlus):
+            return self._affine(e[0])
+        if isinstance(e, ex.PrefixMinus):
+            a = self._affine(e[0])
+            return None if a is None else (a[0], -a[1], -a[2])
+        if isinstance(e, (ex.Plus, ex.Minus)):
+            a, b = self._affine(e[0]), self._affine(e[1])
+            if a is None or b is None:
+                return None
+            if a[0] is not None and b[0] is not None and a[0] != b[0]:
+                return None
+            sg = 1.0 if isinstance(e, ex.Plus) else -1.0
+            return (a[0] or b[0], a[1] + sg * b[1], a[2] + sg * b[2])
+        if isinstance(e, (ex.Multiply, ex.Divide)):
+            a, b = self._affine(e[0]), self._affine(e[1])
+            if a is None or b is None:
+                return None
+            if isinstance(e, ex.Divide):
+                if b[0] is not None or b[2] == 0:
+                    return None
+                return (a[0], a[1] / b[2], a[2] / b[2])
+            if a[0] is None:
+                return (b[0], a[2] * b[1], a[2] * b[2])
+            if b[0] is None:
+                return (a[0], a[1] * b[2], a[2] * b[2])
+            return None
+        return None
 
     def _ex_Log(self, e):
         if len(e) == 1:
@@ -358,8 +428,12 @@ def generate_cell_function(cm, precision, native_maths=False):
             inf, tau = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
             # RL update x = inf - (inf - x) * exp(-dt / tau), as written by
             # the reference (openclsim.cl:358-364)
+            if precision == DOUBLE_PRECISION:
+                arg = '-dt / ' + tau
+            else:
+                arg = '-dt * __builtin_amdgcn_rcpf(' + tau + ')'
             lines.append(tab + 'y[' + str(i) + '] = ' + inf + ' - (' + inf
-                         + ' - ' + x + ') * ' + expf + '(-dt / ' + tau + ');')
+                         + ' - ' + x + ') * ' + expf + '(' + arg + ');')
         else:
             lines.append(tab + 'y[' + str(i) + '] = ' + x + ' + dt * '
                          + _cname(s, 'd_') + ';')
@@ -368,6 +442,24 @@ def generate_cell_function(cm, precision, native_maths=False):
 
 
 _FLOAT_HELPERS = '''
+/* exp(x) in single precision: p = x * log2(e) rounded, e = its rounding
+ * error (exact through fma, plus the tail of log2(e)), then
+ * exp(x) = 2^p * 2^e ~ 2^p * (1 + e * ln 2). One v_exp_f32 and five FMA-class
+ * operations; within 2 ulp (OpenCL allows 3). */
+__device__ __forceinline__ float mk_expf(float x)
+{
+    const float p = x * 0x1.715476p+0f;
+    const float e = __builtin_fmaf(x, 0x1.4ae0c0p-26f,
+                                   __builtin_fmaf(x, 0x1.715476p+0f, -p));
+    const float r = __builtin_amdgcn_exp2f(p);
+    return __builtin_fmaf(r, e * 0x1.62e430p-1f, r);
+}
+
+/* native_maths: 2^(x * log2(e)) directly, like OpenCL's native_exp */
+__device__ __forceinline__ float mk_expf_native(float x)
+{
+    return __builtin_amdgcn_exp2f(x * 0x1.715476p+0f);
+}
 __device__ __forceinline__ float mk_powif(float x, int n)
 {
     float r = 1.0f, b = x;

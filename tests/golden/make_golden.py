@@ -160,6 +160,7 @@ def pacing():
         cases.append({'events': [[e.level(), e.start(), e.duration(),
                                   e.period(), e.multiplier()]
                                  for e in p.events()],
+                      'times': [float(t) for t in times],
                       'sequence': seq, 'error': err})
     with open(os.path.join(HERE, 'pacing_py.json'), 'w') as f:
         json.dump(cases, f)

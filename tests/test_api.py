@@ -1,0 +1,190 @@
+"""
+CPU tests of the SimulationOpenCL front end: argument checks and messages of
+the reference (myokit/tests/test_simulation_opencl.py), state indexing,
+paced-cell geometry, neighbours, log preparation and keys. None of these
+touch the device.
+"""
+import numpy as np
+import pytest
+
+import myokit_amd as myokit
+
+
+@pytest.fixture(scope='module')
+def model():
+    return myokit.load_model('example')
+
+
+def test_creation_errors(model):
+    # test_simulation_opencl.py:530-600
+    with pytest.raises(ValueError, match='scalar or a tuple'):
+        myokit.SimulationOpenCL(model, ncells=(1, 2, 3))
+    with pytest.raises(ValueError, match='at least 1'):
+        myokit.SimulationOpenCL(model, ncells=0)
+    with pytest.raises(ValueError, match='single and double'):
+        myokit.SimulationOpenCL(model, precision=16)
+    m = model.clone()
+    m.get('membrane.V').set_label(None)
+    with pytest.raises(ValueError, match='membrane_potential'):
+        myokit.SimulationOpenCL(m)
+    myokit.SimulationOpenCL(m, diffusion=False)
+
+
+def test_conductance(model):
+    s = myokit.SimulationOpenCL(model, ncells=5)
+    assert s.conductance() == 10
+    s.set_conductance(3, 4)
+    assert s.conductance() == 3
+    s = myokit.SimulationOpenCL(model, ncells=(5, 4))
+    assert s.conductance() == (10, 5)
+    with pytest.raises(ValueError, match='Invalid conductance gx'):
+        s.set_conductance(-1, 1)
+    with pytest.raises(ValueError, match='Invalid conductance gy'):
+        s.set_conductance(1, -1)
+    s = myokit.SimulationOpenCL(model, ncells=5, diffusion=False)
+    with pytest.raises(RuntimeError, match='diffusion is disabled'):
+        s.conductance()
+    with pytest.raises(RuntimeError, match='diffusion is disabled'):
+        s.set_paced_cells()
+
+
+def test_conductance_field_checks(model):
+    s = myokit.SimulationOpenCL(model, ncells=(4, 3))
+    with pytest.raises(ValueError, match='dimensions'):
+        s.set_conductance_field(np.ones((3, 4)), np.ones((2, 4)))
+    with pytest.raises(ValueError, match='must be set'):
+        s.set_conductance_field(np.ones((3, 3)))
+    with pytest.raises(ValueError, match='negative'):
+        s.set_conductance_field(-np.ones((3, 3)), np.ones((2, 4)))
+    s.set_conductance_field(np.ones((3, 3)), np.ones((2, 4)))
+    assert s.conductance() is None
+    s = myokit.SimulationOpenCL(model, ncells=4)
+    with pytest.raises(ValueError, match='length 3'):
+        s.set_conductance_field([1, 2])
+    with pytest.raises(ValueError, match='must be None'):
+        s.set_conductance_field([1, 2, 3], [1])
+
+
+def test_connection_checks(model):
+    # test_simulation_opencl.py:324-392
+    s = myokit.SimulationOpenCL(model, ncells=4)
+    with pytest.raises(ValueError, match='cannot be None'):
+        s.set_connections(None)
+    with pytest.raises(ValueError, match='3-tuples'):
+        s.set_connections([(0, 1)])
+    with pytest.raises(ValueError, match='Invalid connection'):
+        s.set_connections([(0, 4, 1)])
+    with pytest.raises(ValueError, match='Invalid connection'):
+        s.set_connections([(1, 1, 1)])
+    with pytest.raises(ValueError, match='Duplicate connection'):
+        s.set_connections([(0, 1, 1), (1, 0, 2)])
+    with pytest.raises(ValueError, match='Invalid conductance'):
+        s.set_connections([(0, 1, -1)])
+    s.set_connections([(2, 1, 1.5), (0, 1, 2)])
+    assert s._connections == [(1, 2, 1.5), (0, 1, 2.0)]
+    assert sorted(s.neighbors(1)) == [0, 2]
+    s2 = myokit.SimulationOpenCL(model, ncells=(2, 2))
+    with pytest.raises(RuntimeError, match='1d mode'):
+        s2.set_connections([(0, 1, 1)])
+
+
+def test_fields(model):
+    s = myokit.SimulationOpenCL(model, ncells=(3, 2))
+    with pytest.raises(ValueError, match='Bound values'):
+        s.set_field('engine.pace', np.zeros((2, 3)))
+    with pytest.raises(ValueError, match='Only constants'):
+        s.set_field('ina.INa', np.zeros((2, 3)))
+    with pytest.raises(ValueError, match='dimensions'):
+        s.set_field('ina.gNa', np.zeros((3, 2)))
+    s.set_field('ina.gNa', np.ones((2, 3)))
+    s.set_field('ik.IK.gK', np.ones((2, 3)))
+    s.remove_field('ina.gNa')
+    assert list(f.qname() for f in s._fields) == ['ik.IK.gK']
+
+
+def test_paced_cells_geometry(model):
+    # test_simulation_opencl.py:1002-1219
+    s = myokit.SimulationOpenCL(model, ncells=(10, 8))
+    s.set_paced_cells(3, 2, 1, 1)
+    assert s._paced_cells == (3, 2, 1, 1)
+    assert s.is_paced(1, 1) and s.is_paced(3, 2) and not s.is_paced(4, 2)
+    s.set_paced_cells(-3, -2, 10, 8)
+    assert s._paced_cells == (3, 2, 7, 6)
+    s.set_paced_cells(2, 2, -2, -2)
+    assert s._paced_cells == (2, 2, 8, 6)
+    s.set_paced_cell_list([(0, 0), (9, 7)])
+    assert s._paced_cells == [0, 79]
+    assert s.is_paced(9, 7) and not s.is_paced(8, 7)
+    with pytest.raises(IndexError):
+        s.set_paced_cell_list([(10, 0)])
+    with pytest.raises(ValueError, match='No y-coordinate'):
+        s.is_paced(1)
+    s = myokit.SimulationOpenCL(model, ncells=10)
+    s.set_paced_cells(3, 4, 5, 6)
+    assert s._paced_cells == (3, 1, 5, 0)
+    with pytest.raises(ValueError, match='Y-coordinate'):
+        s.is_paced(1, 1)
+
+
+def test_neighbors(model):
+    s = myokit.SimulationOpenCL(model, ncells=(3, 3))
+    assert s.neighbors(0, 0) == [(1, 0), (0, 1)]
+    assert s.neighbors(1, 1) == [(0, 1), (2, 1), (1, 0), (1, 2)]
+    s = myokit.SimulationOpenCL(model, ncells=3)
+    assert s.neighbors(0) == [1] and s.neighbors(1) == [0, 2]
+    with pytest.raises(IndexError):
+        s.neighbors(3)
+
+
+def test_state_indexing_x_first(model):
+    # test_simulation_opencl.py:1221-1410
+    s = myokit.SimulationOpenCL(model, ncells=(3, 2))
+    n = model.count_states()
+    full = np.arange(6 * n, dtype=float)
+    s.set_state(full)
+    assert s.state(1, 1) == list(full[4 * n:5 * n])
+    s.set_state(list(range(n)), 2, 0)
+    assert s.state(2, 0) == list(range(n))
+    with pytest.raises(ValueError, match='argument x'):
+        s.set_state(full, 1)
+    with pytest.raises(ValueError, match='same size'):
+        s.set_state([1, 2])
+    with pytest.raises(IndexError):
+        s.state(3, 0)
+    s.set_default_state(model.state())
+    assert s.default_state(0, 1) == model.state()
+    assert s.shape() == (2, 3)
+    with pytest.raises(ValueError, match='negative'):
+        s.run(-1)
+
+
+def test_prepare_log_keys(model):
+    m = model
+    d = myokit.prepare_log(['engine.time', 'membrane.V'], m, dims=(3, 2),
+                           global_vars=['engine.time', 'engine.pace'])
+    assert list(d.keys()) == ['engine.time', '0.0.membrane.V',
+                              '1.0.membrane.V', '2.0.membrane.V',
+                              '0.1.membrane.V', '1.1.membrane.V',
+                              '2.1.membrane.V']
+    assert d.time_key() == 'engine.time'
+    d = myokit.prepare_log(myokit.LOG_STATE + myokit.LOG_BOUND, m, dims=(2,),
+                           global_vars=['engine.time', 'engine.pace'])
+    assert '1.ica.Ca_i' in d and 'engine.pace' in d and '0.membrane.i_diff' in d
+    with pytest.raises(ValueError, match='Unknown variable'):
+        myokit.prepare_log(['x.y'], m, dims=(2,))
+    with pytest.raises(ValueError, match='constants'):
+        myokit.prepare_log(['ina.gNa'], m, dims=(2,))
+    with pytest.raises(ValueError, match='Invalid index'):
+        myokit.prepare_log(['5.membrane.V'], m, dims=(2,))
+    with pytest.raises(ValueError, match='global'):
+        myokit.prepare_log(['0.engine.time'], m, dims=(2,),
+                           global_vars=['engine.time'])
+    assert myokit.split_key('12.3.membrane.V') == ('12.3.', 'membrane.V')
+    assert myokit.split_key('membrane.V') == ('', 'membrane.V')
+
+
+def test_monodomain_conductance(model):
+    s = myokit.SimulationOpenCL(model, ncells=3)
+    # g = D * A * k / ((k + 1) * chi * dx^2)
+    assert s.monodomain_conductance(2, 3, 4, 0.5, 1) == pytest.approx(
+        4 * 3 / (4 * 2 * 0.25))
