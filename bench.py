@@ -130,12 +130,13 @@ def main():
                                               steps_per_seg),
             'cells': n, 'time_steps_per_step': steps_per_seg,
             'precision': args.precision,
+            'driver': s.engine_driver(),
         },
         'roofline': {
             'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
             'traffic': None,
-            'kernel': 'mk_grid', 'avg_launch_ms': avg_ms,
+            'kernel': 'mk_' + s.engine_driver(), 'avg_launch_ms': avg_ms,
             'bytes_per_cell_step': bytes_per_cell_step,
             'cells_per_launch': cells_per_launch,
         },

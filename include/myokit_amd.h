@@ -57,6 +57,7 @@ extern "C" {
 #define MKH_DRIVER_CELLS 0  /* one thread per uncoupled cell, all steps per launch */
 #define MKH_DRIVER_BLOCK 1  /* one workgroup per domain, all steps per launch */
 #define MKH_DRIVER_GRID 2   /* whole grid, one fused step per launch */
+#define MKH_DRIVER_SHEET 3  /* 2D sheet, persistent tiles, all steps per launch */
 
 /* Simulation geometry and kernel layout. */
 typedef struct mkh_sim_config {
@@ -70,7 +71,8 @@ typedef struct mkh_sim_config {
     int block_threads;     /* threads per workgroup */
     long long nx, ny;      /* grid size (1D: ny = 1) */
     int device;            /* HIP device ordinal */
-    int reserved;
+    int sheet_cpt;         /* cells per thread of the sheet driver (MK_SCPT in
+                              the kernel source), 0 = no sheet driver */
 } mkh_sim_config;
 
 /* Library */
@@ -167,6 +169,9 @@ int mkh_sim_set_option(void* sim, const char* key, long long value);
 int mkh_sim_get_timing(void* sim, double* kernel_ms, long long* launches,
                        long long* cell_steps);
 int mkh_sim_reset_timing(void* sim);
+
+/* The execution driver (MKH_DRIVER_*) the simulation currently uses. */
+int mkh_sim_driver(void* sim, int* driver);
 
 /* Number of log rows produced so far in the current run. */
 int mkh_sim_log_rows(void* sim, long long* rows);

@@ -428,8 +428,14 @@ def generate_cell_function(cm, precision, native_maths=False):
             inf, tau = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
             # RL update x = inf - (inf - x) * exp(-dt / tau), as written by
             # the reference (openclsim.cl:358-364)
+            trhs = rl[s][1].rhs()
             if precision == DOUBLE_PRECISION:
                 arg = '-dt / ' + tau
+            elif isinstance(trhs, ex.Divide) and isinstance(
+                    trhs[0], ex.Number) and trhs[0].eval() == 1:
+                # tau = 1 / X (the form the RL conversion writes), so
+                # -dt / tau = -dt * X without two reciprocals
+                arg = '-dt * ' + w.ex(trhs[1])
             else:
                 arg = '-dt * __builtin_amdgcn_rcpf(' + tau + ')'
             lines.append(tab + 'y[' + str(i) + '] = ' + inf + ' - (' + inf
@@ -475,6 +481,24 @@ __device__ __forceinline__ float mk_powif(float x, int n)
 '''
 
 
+def sheet_cpt(precision):
+    """Cells per thread of the persistent sheet driver (4: a tile is 64 x 16
+    cells for a 256-thread workgroup)."""
+    return 4
+
+
+def sheet_waves(precision):
+    """Waves per SIMD the sheet kernel's register budget must allow: 4 in
+    single precision (128 VGPRs, 4096 cells per CU, so 256 CUs hold 1024^2),
+    2 in double (256 VGPRs; sheets up to half that size, larger ones use the
+    grid driver)."""
+    return 2 if precision == DOUBLE_PRECISION else 4
+
+
+def sheet_enabled(precision):
+    return True
+
+
 def generate_kernel(cm, precision, coupling, block_threads=256,
                     native_maths=False):
     """Returns the complete HIP source for hiprtc."""
@@ -495,6 +519,8 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_COUPLING ' + str(int(coupling)),
         '#define MK_BT ' + str(int(block_threads)),
         '#define MK_BCELLS ' + str(bcells),
+        '#define MK_SCPT ' + str(sheet_cpt(precision)),
+        '#define MK_SWAVES ' + str(sheet_waves(precision)),
         'typedef ' + real + ' Real;',
         _read('mk_math.h'),
         _read('mk_args.h'),

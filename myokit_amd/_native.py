@@ -25,6 +25,7 @@ COUPLING_GRAPH = 3
 DRIVER_CELLS = 0
 DRIVER_BLOCK = 1
 DRIVER_GRID = 2
+DRIVER_SHEET = 3
 
 # Every entry point declared in include/myokit_amd.h
 _c = ctypes
@@ -42,7 +43,7 @@ class SimConfig(ctypes.Structure):
         ('precision', _i), ('n_state', _i), ('n_field', _i),
         ('n_quant', _i), ('i_vm', _i), ('coupling', _i), ('driver', _i),
         ('block_threads', _i), ('nx', _ll), ('ny', _ll), ('device', _i),
-        ('reserved', _i),
+        ('sheet_cpt', _i),
     ]
 
 
@@ -76,6 +77,7 @@ SIGNATURES = {
     'mkh_sim_get_timing': (_i, [_vp, _pd, _pll, _pll]),
     'mkh_sim_reset_timing': (_i, [_vp]),
     'mkh_sim_log_rows': (_i, [_vp, _pll]),
+    'mkh_sim_driver': (_i, [_vp, _pi]),
     'mkh_pacing_levels': (_i, [_pd, _ll, _d, _pd, _ll, _pd, _pd]),
     'mkh_plan': (_i, [_d, _d, _d, _pd, _ll, _d, _pd, _pd, _pd, _pi, _ll,
                       _pll]),

@@ -47,4 +47,16 @@ struct MkArgsT {
     R gx, gy;                   /* homogeneous conductances */
 };
 
+/* Persistent sheet driver: neighbour exchange between workgroup tiles. */
+template <typename R>
+struct MkSheetT {
+    R* edges;         /* [2][tiles][4][64] tile edge potentials (by parity) */
+    int* cnt;         /* [tiles] steps completed by each tile */
+    int* err;         /* set to 1 if a neighbour wait timed out */
+    int tiles_x;      /* tiles along x (64 cells each) */
+    int tiles_y;      /* tiles along y (MK_STH cells each) */
+    int g0;           /* steps completed before this launch */
+    int spin_limit;   /* polls before a wait is abandoned */
+};
+
 #endif

@@ -147,14 +147,12 @@ mk_cells(const MkArgs a, const int s0, const int s1)
     for (int s = s0; s < s1; ++s) {
         const MkStep p = a.steps[s];
         const Real pace = paced ? p.pace : (Real)0;
-        if (p.row >= 0) {
-            if (a.nan_check && c == 0 && y[0] != y[0])
-                __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
-            mk_cell<true>(y, p.t, p.dt, pace, (Real)0, fv, q);
-            mk_write_log(a, c, p.row, q);
-        } else {
-            mk_cell<false>(y, p.t, p.dt, pace, (Real)0, fv, q);
-        }
+        if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
+            __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+        /* One instance of the cell code: logged values are register
+         * copies, written out only at log steps */
+        mk_cell<true>(y, p.t, p.dt, pace, (Real)0, fv, q);
+        if (p.row >= 0) mk_write_log(a, c, p.row, q);
     }
 #pragma unroll
     for (int i = 0; i < MK_NS; ++i) a.state_a[(long long)i * a.npad + c] = y[i];
@@ -208,14 +206,12 @@ mk_block(const MkArgs a, const int s0, const int s1)
             const Real idiff = (Real)0;
 #endif
             const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace : (Real)0;
-            if (p.row >= 0) {
-                if (a.nan_check && c == 0 && y[0] != y[0])
-                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
-                mk_cell<true>(y, p.t, p.dt, pace, idiff, fv, q);
-                mk_write_log(a, c, p.row, q);
-            } else {
-                mk_cell<false>(y, p.t, p.dt, pace, idiff, fv, q);
-            }
+            if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
+                __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+            /* One instance of the cell code: logged values are register
+             * copies, written out only at log steps */
+            mk_cell<true>(y, p.t, p.dt, pace, idiff, fv, q);
+            if (p.row >= 0) mk_write_log(a, c, p.row, q);
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i)
                 a.state_a[(long long)i * a.npad + c] = y[i];
@@ -285,14 +281,245 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
     const Real idiff = (Real)0;
 #endif
     const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace : (Real)0;
-    if (p.row >= 0) {
-        if (a.nan_check && c == 0 && y[0] != y[0])
-            __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
-        mk_cell<true>(y, p.t, p.dt, pace, idiff, fv, q);
-        mk_write_log(a, c, p.row, q);
-    } else {
-        mk_cell<false>(y, p.t, p.dt, pace, idiff, fv, q);
-    }
+    if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
+        __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+    /* One instance of the cell code: logged values are register
+     * copies, written out only at log steps */
+    mk_cell<true>(y, p.t, p.dt, pace, idiff, fv, q);
+    if (p.row >= 0) mk_write_log(a, c, p.row, q);
 #pragma unroll
     for (int i = 0; i < MK_NS; ++i) dst[(long long)i * a.npad + c] = y[i];
+}
+
+/* ------------------------------------------------------------------------
+ * Persistent 2D sheet: register-resident state, LDS halos, neighbour sync.
+ *
+ * Each workgroup (256 threads = 4 waves) owns a tile of 64 x MK_STH cells
+ * (MK_STH = 4 * MK_SCPT) for every step of the launch; thread (lx, w) keeps
+ * the full state of cells (x0 + lx, y0 + w * MK_SCPT + k), k < MK_SCPT, in
+ * registers. The tile's membrane potentials, plus a one-cell halo ring, live
+ * in LDS (double-buffered by step). After each step a tile writes its four
+ * edge rows/columns of V to global memory with agent-scope atomic stores,
+ * waits for their completion and publishes its step count (agent-scope
+ * atomic store); before the next step it polls the step counts of its four
+ * neighbours and copies their edges (agent-scope atomic loads) into its halo
+ * ring. No state goes to HBM inside a launch: per cell-step the only global
+ * traffic is the edge exchange.
+ *
+ * The launch is cooperative, so all tiles are resident at once. Every wait
+ * is bounded (spin_limit polls); on a timeout the tile records an error and
+ * returns, its neighbours then time out too, and every wave exits.
+ * ---------------------------------------------------------------------- */
+#ifndef MK_SCPT
+#define MK_SCPT 4
+#endif
+#ifndef MK_SWAVES
+#define MK_SWAVES 4     /* waves per SIMD the register budget must allow */
+#endif
+#define MK_STH (4 * MK_SCPT)
+typedef MkSheetT<Real> MkSheet;
+
+/* Exchanged values are written and read with agent-scope atomic accesses:
+ * these are coherent across the XCDs' separate L2 caches (write-through on
+ * store, L2 miss on load), so the exchange needs no L2-wide write-back or
+ * invalidation, which as a per-step fence cost more than the step itself. */
+__device__ __forceinline__ void mk_xstore(Real* p, Real v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ Real mk_xload(const Real* p)
+{
+    return __hip_atomic_load(const_cast<Real*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ Real* mk_edge(const MkSheet& sh, int par, int tile,
+                                         int e)
+{
+    return sh.edges + ((long long)(par * sh.tiles_x * sh.tiles_y + tile) * 4
+                       + e) * 64;
+}
+
+extern "C" __global__ void __launch_bounds__(256, MK_SWAVES)
+mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
+{
+    __shared__ Real vt[2][MK_STH + 2][66];
+    __shared__ int wg_flag;
+    if (threadIdx.x == 0)
+        wg_flag = mk_halted(a, a.steps[s1 - 1].row) ? 1 : 0;
+    __syncthreads();
+    if (wg_flag) return;
+
+    const int tile = blockIdx.x;
+    const int tx = tile % sh.tiles_x, ty = tile / sh.tiles_x;
+    const int lx = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long gx0 = (long long)tx * 64, gy0 = (long long)ty * MK_STH;
+    const long long ix = gx0 + lx;
+    const long long nx = a.nx, ny = a.ny;
+
+    Real y[MK_SCPT][MK_NS];
+    Real fv[MK_SCPT][MK_NFA];
+    Real q[MK_NQA];
+    bool act[MK_SCPT];
+    bool paced[MK_SCPT];
+#pragma unroll
+    for (int k = 0; k < MK_SCPT; ++k) {
+        const int r = w * MK_SCPT + k;
+        const long long iy = gy0 + r;
+        act[k] = ix < nx && iy < ny;
+        const long long c = act[k] ? ix + iy * nx : 0;
+        paced[k] = act[k] && mk_is_paced(a, c, ix, iy);
+#pragma unroll
+        for (int i = 0; i < MK_NS; ++i)
+            y[k][i] = act[k] ? a.state_a[(long long)i * a.npad + c] : (Real)0;
+#pragma unroll
+        for (int f = 0; f < MK_NF; ++f)
+            fv[k][f] = act[k] ? a.fields[(long long)f * a.npad + c] : (Real)0;
+        vt[0][r + 1][lx + 1] = y[k][MK_IVM];
+    }
+    /* Halo ring at the start of the launch, from the global state */
+    {
+        const Real* V = a.state_a + (long long)MK_IVM * a.npad;
+        const int t = threadIdx.x;
+        long long hx = -1, hy = -1;
+        int lr = 0, lc = 0;
+        if (t < 66) {
+            hx = gx0 - 1 + t; hy = gy0 - 1; lr = 0; lc = t;
+        } else if (t < 132) {
+            hx = gx0 - 1 + (t - 66); hy = gy0 + MK_STH; lr = MK_STH + 1;
+            lc = t - 66;
+        } else if (t < 132 + MK_STH) {
+            hx = gx0 - 1; hy = gy0 + (t - 132); lr = t - 132 + 1; lc = 0;
+        } else if (t < 132 + 2 * MK_STH) {
+            hx = gx0 + 64; hy = gy0 + (t - 132 - MK_STH);
+            lr = t - 132 - MK_STH + 1; lc = 65;
+        }
+        if (hx >= 0 && hy >= 0 && hx < nx && hy < ny)
+            vt[0][lr][lc] = V[hx + hy * nx];
+    }
+    __syncthreads();
+
+    const int ntiles = sh.tiles_x * sh.tiles_y;
+    const int nb_tile[4] = {ty > 0 ? tile - sh.tiles_x : -1,
+                            ty + 1 < sh.tiles_y ? tile + sh.tiles_x : -1,
+                            tx > 0 ? tile - 1 : -1,
+                            tx + 1 < sh.tiles_x ? tile + 1 : -1};
+    (void)ntiles;
+    int cur = 0;
+    for (int s = s0; s < s1; ++s) {
+        const int i = s - s0;
+        if (i > 0) {
+            /* Wait for the neighbours' previous step, then read their edges
+             * (south: its top row, north: its bottom row, west: its right
+             * column, east: its left column) into the halo ring. */
+            const int target = sh.g0 + i;
+            if (threadIdx.x < 4) {
+                const int nbt = nb_tile[threadIdx.x];
+                if (nbt >= 0) {
+                    int spins = 0;
+                    while (__hip_atomic_load(sh.cnt + nbt, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)
+                           < target) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > sh.spin_limit) {
+                            __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            wg_flag = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (wg_flag) return;
+            const int par = (sh.g0 + i - 1) & 1;
+            const int t = threadIdx.x;
+            if (t < 64) {
+                if (nb_tile[0] >= 0)
+                    vt[cur][0][t + 1] =
+                        mk_xload(mk_edge(sh, par, nb_tile[0], 1) + t);
+            } else if (t < 128) {
+                if (nb_tile[1] >= 0)
+                    vt[cur][MK_STH + 1][t - 63] =
+                        mk_xload(mk_edge(sh, par, nb_tile[1], 0) + t - 64);
+            } else if (t < 128 + MK_STH) {
+                if (nb_tile[2] >= 0)
+                    vt[cur][t - 127][0] =
+                        mk_xload(mk_edge(sh, par, nb_tile[2], 3) + t - 128);
+            } else if (t < 128 + 2 * MK_STH) {
+                if (nb_tile[3] >= 0)
+                    vt[cur][t - 127 - MK_STH][65] = mk_xload(
+                        mk_edge(sh, par, nb_tile[3], 2) + t - 128 - MK_STH);
+            }
+            __syncthreads();
+        }
+        const MkStep p = a.steps[s];
+        const bool last = s + 1 == s1;
+        const int par = (sh.g0 + i) & 1;
+#pragma unroll
+        for (int k = 0; k < MK_SCPT; ++k) {
+            const int r = w * MK_SCPT + k;
+            if (act[k]) {
+                const long long iy = gy0 + r;
+                const Real v = vt[cur][r + 1][lx + 1];
+                Real id;
+                if (nx > 1) {
+                    if (ix == 0) id = a.gx * (v - vt[cur][r + 1][lx + 2]);
+                    else if (ix == nx - 1) id = a.gx * (v - vt[cur][r + 1][lx]);
+                    else id = a.gx * (((Real)2 * v - vt[cur][r + 1][lx])
+                                      - vt[cur][r + 1][lx + 2]);
+                } else {
+                    id = (Real)0;
+                }
+                if (ny > 1) {
+                    if (iy == 0) id += a.gy * (v - vt[cur][r + 2][lx + 1]);
+                    else if (iy == ny - 1) id += a.gy * (v - vt[cur][r][lx + 1]);
+                    else id += a.gy * (((Real)2 * v - vt[cur][r][lx + 1])
+                                       - vt[cur][r + 2][lx + 1]);
+                }
+                const Real pace = paced[k] ? p.pace : (Real)0;
+                const long long c = ix + iy * nx;
+                if (p.row >= 0 && a.nan_check && c == 0 && y[k][0] != y[k][0])
+                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+                /* One instance of the cell code: logged values are register
+                 * copies, written out only at log steps */
+                mk_cell<true>(y[k], p.t, p.dt, pace, id, fv[k], q);
+                if (p.row >= 0) mk_write_log(a, c, p.row, q);
+                const Real vn = y[k][MK_IVM];
+                vt[cur ^ 1][r + 1][lx + 1] = vn;
+                if (!last) {
+                    if (r == 0) mk_xstore(mk_edge(sh, par, tile, 0) + lx, vn);
+                    if (r == MK_STH - 1)
+                        mk_xstore(mk_edge(sh, par, tile, 1) + lx, vn);
+                    if (lx == 0) mk_xstore(mk_edge(sh, par, tile, 2) + r, vn);
+                    if (lx == 63) mk_xstore(mk_edge(sh, par, tile, 3) + r, vn);
+                }
+            }
+        }
+        if (!last) {
+            /* Every wave waits until its edge stores are complete (written
+             * through to memory at agent scope), then one thread publishes
+             * the step count. */
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(sh.cnt + tile, sh.g0 + i + 1,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        cur ^= 1;
+    }
+#pragma unroll
+    for (int k = 0; k < MK_SCPT; ++k) {
+        if (act[k]) {
+            const long long c = ix + (gy0 + w * MK_SCPT + k) * nx;
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                a.state_a[(long long)i * a.npad + c] = y[k][i];
+        }
+    }
+    /* The final step count is published with the state at kernel end */
+    if (threadIdx.x == 0)
+        __hip_atomic_store(sh.cnt + tile, sh.g0 + (s1 - s0), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -59,6 +59,7 @@ struct Program {
     int device = -1;
     hipModule_t module = nullptr;
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
+    hipFunction_t f_sheet = nullptr;
     std::vector<char> code;
 };
 
@@ -208,6 +209,7 @@ struct SimBase {
     virtual int timing(double*, long long*, long long*) = 0;
     virtual int reset_timing() = 0;
     virtual long long log_rows() = 0;
+    virtual int get_driver() = 0;
 };
 
 template <typename R>
@@ -230,6 +232,11 @@ struct Sim : SimBase {
     long long nquant = 0;
     int driver = MKH_DRIVER_GRID;
     long long ndom = 1;
+    /* sheet driver */
+    DevBuf<R> sh_edges;
+    DevBuf<int> sh_cnt, sh_err;
+    int sh_tx = 0, sh_ty = 0, sh_capacity = 0;
+    long long sh_steps = 0;
     long long max_domain = 0;
     int max_block_cells = 0;
 
@@ -293,6 +300,25 @@ struct Sim : SimBase {
         ndom = 1;
         max_domain = n;
         pace_mode = 0;
+        if (cfg.sheet_cpt > 0 && cfg.ny > 1 && p->f_sheet) {
+            int per_cu = 0, dev = cfg.device;
+            MKH_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, p->f_sheet, 256, 0));
+            hipDeviceProp_t prop;
+            MKH_HIP(hipGetDeviceProperties(&prop, dev));
+            sh_capacity = per_cu * prop.multiProcessorCount;
+            const int th = 4 * cfg.sheet_cpt;
+            sh_tx = (int)((cfg.nx + 63) / 64);
+            sh_ty = (int)((cfg.ny + th - 1) / th);
+            if ((long long)sh_tx * sh_ty <= sh_capacity) {
+                const size_t nt = (size_t)sh_tx * sh_ty;
+                MKH_HIP(sh_edges.alloc(2 * nt * 4 * 64));
+                MKH_HIP(sh_cnt.alloc(nt));
+                MKH_HIP(sh_err.alloc(1));
+                MKH_HIP(hipMemset(sh_cnt.p, 0, nt * sizeof(int)));
+                MKH_HIP(hipMemset(sh_err.p, 0, sizeof(int)));
+            }
+        }
         choose_driver();
         return 0;
     }
@@ -305,6 +331,10 @@ struct Sim : SimBase {
         }
         if (cfg.coupling == MKH_COUPLING_NONE) {
             driver = MKH_DRIVER_CELLS;
+        } else if (cfg.coupling == MKH_COUPLING_STENCIL && cfg.ny > 1 &&
+                   n > 4096 && sh_tx > 0 &&
+                   (long long)sh_tx * sh_ty <= sh_capacity) {
+            driver = MKH_DRIVER_SHEET;
         } else if (max_domain <= max_block_cells &&
                    (ndom >= 64 || max_domain <= 512)) {
             driver = MKH_DRIVER_BLOCK;
@@ -622,6 +652,11 @@ struct Sim : SimBase {
             MKH_HIP(logbuf.alloc((size_t)cap * ncols));
         }
         log_rows_cap_run = cap;
+        if (sh_cnt.p) {
+            MKH_HIP(hipMemset(sh_cnt.p, 0, sh_cnt.n * sizeof(int)));
+            MKH_HIP(hipMemset(sh_err.p, 0, sizeof(int)));
+            sh_steps = 0;
+        }
         size_t need = (size_t)batch_steps;
         MKH_HIP(steps.alloc(need));
         if (h_steps_cap < need) {
@@ -753,6 +788,36 @@ struct Sim : SimBase {
                 if (e) return e;
                 cur ^= 1;
             }
+        } else if (driver == MKH_DRIVER_SHEET) {
+            Args a = base_args();
+            MkSheetT<R> sh;
+            sh.edges = sh_edges.p;
+            sh.cnt = sh_cnt.p;
+            sh.err = sh_err.p;
+            sh.tiles_x = sh_tx;
+            sh.tiles_y = sh_ty;
+            sh.spin_limit = 1 << 22;
+            const unsigned ntiles = (unsigned)(sh_tx * sh_ty);
+            long long s0 = 0;
+            for (long long s = 0; s < ns; ++s) {
+                if (h_steps[s].row >= 0 || s == ns - 1) {
+                    int i0 = (int)s0, i1 = (int)(s + 1);
+                    sh.g0 = (int)sh_steps;
+                    void* params[] = {&a, &sh, &i0, &i1};
+                    if (time_kernels)
+                        MKH_HIP(hipEventRecord(event(nev), stream));
+                    MKH_HIP(hipModuleLaunchCooperativeKernel(
+                        prog->f_sheet, ntiles, 1, 1, 256, 1, 1, 0, stream,
+                        params));
+                    if (time_kernels) {
+                        MKH_HIP(hipEventRecord(event(nev + 1), stream));
+                        nev += 2;
+                    }
+                    launches++;
+                    sh_steps += i1 - i0;
+                    s0 = s + 1;
+                }
+            }
         } else {
             hipFunction_t f = driver == MKH_DRIVER_CELLS ? prog->f_cells
                                                          : prog->f_block;
@@ -777,6 +842,16 @@ struct Sim : SimBase {
         MKH_HIP(hipMemcpyAsync(&h, halt.p, sizeof(int), hipMemcpyDeviceToHost,
                                stream));
         MKH_HIP(hipStreamSynchronize(stream));
+        if (driver == MKH_DRIVER_SHEET) {
+            int e = 0;
+            MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
+                              hipMemcpyDeviceToHost));
+            if (e) {
+                running = false;
+                return fail(MKH_E_HIP, "sheet driver: a neighbour exchange "
+                                       "timed out");
+            }
+        }
         long long valid = nr;
         if (h >= 0) valid = h + 1;
         if (nr > 0 && ncols > 0 && dst_values)
@@ -852,6 +927,7 @@ struct Sim : SimBase {
     }
 
     long long log_rows() override { return rows_done; }
+    int get_driver() override { return driver; }
 };
 
 }  // namespace
@@ -966,6 +1042,7 @@ int mkh_program_build(const char* source, const char* options,
         MKH_HIP(hipModuleGetFunction(&prog->f_cells, prog->module, "mk_cells"));
         MKH_HIP(hipModuleGetFunction(&prog->f_block, prog->module, "mk_block"));
         MKH_HIP(hipModuleGetFunction(&prog->f_grid, prog->module, "mk_grid"));
+        MKH_HIP(hipModuleGetFunction(&prog->f_sheet, prog->module, "mk_sheet"));
     }
     *program = prog.release();
     return 0;
@@ -1144,6 +1221,13 @@ int mkh_sim_reset_timing(void* sim)
 {
     CHECK_SIM();
     return SIM->reset_timing();
+}
+
+int mkh_sim_driver(void* sim, int* driver)
+{
+    CHECK_SIM();
+    if (driver) *driver = SIM->get_driver();
+    return 0;
 }
 
 int mkh_sim_log_rows(void* sim, long long* rows)

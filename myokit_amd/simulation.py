@@ -628,6 +628,16 @@ class SimulationOpenCL:
             _native.check(_native.lib().mkh_sim_set_option(
                 self._engine.h, key.encode(), int(value)), 'set_option')
 
+    def engine_driver(self):
+        """Name of the device execution driver of the last run (``cells``,
+        ``block``, ``grid`` or ``sheet``), or None before the first run."""
+        if self._engine is None:
+            return None
+        d = ctypes.c_int(-1)
+        _native.check(_native.lib().mkh_sim_driver(self._engine.h,
+                                                   ctypes.byref(d)), 'driver')
+        return {0: 'cells', 1: 'block', 2: 'grid', 3: 'sheet'}[d.value]
+
     def engine_timing(self, reset=False):
         """Kernel timing of the device engine (needs the ``time_kernels``
         option): ``{'kernel_ms', 'launches', 'cell_steps'}``."""
@@ -675,6 +685,8 @@ class SimulationOpenCL:
         cfg.nx = self._nx
         cfg.ny = self._ny
         cfg.device = self._device
+        cfg.sheet_cpt = _kernelgen.sheet_cpt(self._precision) \
+            if _kernelgen.sheet_enabled(self._precision) else 0
         eng = _Engine(prog, cfg)
         for k, v in self._options.items():
             _native.check(_native.lib().mkh_sim_set_option(

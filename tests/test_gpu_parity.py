@@ -224,3 +224,64 @@ def test_nan_raises_simulation_error():
     s.set_step_size(5)
     with pytest.raises(myokit.SimulationError):
         s.run(200, log=['engine.time', 'membrane.V'], log_interval=5)
+
+
+@pytest.mark.parametrize('shape', [(128, 96), (100, 70)])
+def test_fp32_persistent_sheet(shape):
+    # Large enough for the persistent sheet driver (register-resident tiles
+    # with neighbour exchange); the second shape has partial edge tiles
+    sc = Scenario('lr1991', shape, rl=True, dt=0.01, gx=9.5, gy=9.5,
+                  events=[(1, 0, 2, 1000, 0)], paced=(4, shape[1], 0, 0))
+    P, O = run_both(sc, 20.0, myokit.SINGLE_PRECISION, log_interval=0.25)
+    assert P['sim'].engine_driver() == 'sheet'
+    rms = np.sqrt(np.mean((P['V'] - O['V']) ** 2))
+    ta = activation_times(P['time'], P['V'])
+    tb = activation_times(O['time'], O['V'])
+    ok = np.isfinite(tb)
+    assert ok.sum() > 100
+    dta = np.max(np.abs(ta[ok] - tb[ok]))
+    print('sheet %s: V RMS %.3g mV, max activation dt %.3g ms'
+          % (shape, rms, dta), flush=True)
+    assert rms <= 0.5
+    assert dta <= 0.1
+    # Same run on the one-step-per-launch grid driver
+    g = sc.product(myokit.SINGLE_PRECISION)
+    g.set_engine_option('driver', 2)
+    d = g.run(20.0, log=['membrane.V'], log_interval=0.25)
+    assert g.engine_driver() == 'grid'
+    nx = shape[0]
+    Vg = np.stack([np.asarray(d['%d.%d.membrane.V' % (c % nx, c // nx)])
+                   for c in range(nx * shape[1])], axis=1)
+    dd = np.max(np.abs(Vg - P['V']))
+    rg = np.sqrt(np.mean((Vg - P['V']) ** 2))
+    print('sheet vs grid max|dV| %.3g mV, RMS %.3g mV' % (dd, rg), flush=True)
+    # Both are single-precision runs; fma contraction may differ between the
+    # two kernels, so they agree to single-precision tolerance only
+    assert rg <= 0.5 and dd < 1.0
+
+
+def test_fp32_sheet_continues_across_runs():
+    # State stays on the device between runs; two 10 ms runs equal one
+    # 20 ms run
+    shape = (128, 96)
+    sc = Scenario('lr1991', shape, rl=True, dt=0.01, gx=9.5, gy=9.5,
+                  events=[(1, 0, 2, 1000, 0)], paced=(4, shape[1], 0, 0))
+    a = sc.product(32)
+    b = sc.product(32)
+    a.run(20.0, log=myokit.LOG_NONE)
+    b.run(10.0, log=myokit.LOG_NONE)
+    b.run(10.0, log=myokit.LOG_NONE)
+    assert a.engine_driver() == 'sheet'
+    np.testing.assert_array_equal(a.state(), b.state())
+
+
+@pytest.mark.parametrize('shape', [(128, 96), (200, 40)])
+def test_fp64_persistent_sheet_bit_exact(shape):
+    # The sheet driver's neighbour exchange, checked bit for bit in double
+    # precision: any stale halo value would show as a difference
+    sc = Scenario('lr1991', shape, rl=True, dt=0.01, gx=9.5, gy=7.0,
+                  events=[(1, 0, 2, 1000, 0)], paced=(4, 20, 0, 10))
+    P, O = run_both(sc, 12.0, myokit.DOUBLE_PRECISION, log_interval=0.5)
+    assert P['sim'].engine_driver() == 'sheet'
+    assert _report('fp64 sheet ' + str(shape), P, O) <= FP64_TOL
+    assert np.max(np.abs(P['state'] - O['state'])) <= FP64_TOL
