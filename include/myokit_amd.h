@@ -170,6 +170,33 @@ int mkh_sim_get_timing(void* sim, double* kernel_ms, long long* launches,
                        long long* cell_steps);
 int mkh_sim_reset_timing(void* sim);
 
+/* Discrete adjoint of a run. No counterpart in the reference (myokit has no
+ * gradients); for callers that fit per-cell fields and conductances to
+ * recorded potentials. Runs tmin..tmax from the simulation's current state
+ * (left unchanged) with the loss
+ *     L = sum_{s=1..N} sum_p (V(t_s)[probes[p]] - target[(s-1) * n_probes + p])^2
+ * over the N time steps of the run (one target row per step), and returns L,
+ * dL/dfield as grad_fields[k * n_cells + c], and dL/dgx, dL/dgy (graphs:
+ * dL/ds for all conductances scaled by s) in grad_g[0..1]. The kernel must
+ * be generated with the adjoint; state is checkpointed every
+ * checkpoint_every steps and recomputed segment by segment; loss and
+ * gradients accumulate in double precision. */
+int mkh_adj_run(void* sim, double tmin, double tmax, double dt,
+                const double* events, long long n_events,
+                const long long* probes, long long n_probes,
+                const double* target, long long n_target_rows,
+                int checkpoint_every, double* loss, double* grad_fields,
+                double* grad_g);
+
+/* The forward quantity of mkh_adj_run's loss: V at the probe cells after
+ * each of the N time steps of tmin..tmax, into trace[s * n_probes + p]
+ * (n_rows >= N). Leaves the state unchanged. No counterpart in the
+ * reference; needs a kernel generated with the adjoint. */
+int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,
+                  const double* events, long long n_events,
+                  const long long* probes, long long n_probes, double* trace,
+                  long long n_rows);
+
 /* The execution driver (MKH_DRIVER_*) the simulation currently uses. */
 int mkh_sim_driver(void* sim, int* driver);
 

@@ -1,0 +1,266 @@
+"""
+Discrete adjoint (vector-Jacobian product) of the generated cell step.
+
+The reference has no adjoint; this builds one from the model's expression
+tree, in the spirit of myokit's symbolic derivatives (``Expression.diff``,
+``myokit/_expressions.py:243-336``) but in reverse mode. For one cell and one
+step ``y' = F(y, idiff, fields)`` (the forward-Euler / Rush-Larsen update of
+``_sim/openclsim.cl:358-364``), ``mk_cell_vjp`` returns, for a given
+adjoint ``lam`` of ``y'``:
+
+    mu      = (dF/dy)^T lam        (adjoint of the cell's own state)
+    abar    = (dF/didiff)^T lam    (adjoint of the diffusion current)
+    fbar[k] = (dF/dfield_k)^T lam
+
+It recomputes the forward sweep, then walks the equations in reverse
+dependency order. Each equation's expression tree is differentiated with the
+ordinary rules of calculus. Contributions that flow through a branch of
+``if``/``piecewise`` are added as ``cond ? contribution : 0``, so values of
+the branch not taken (which may be inf or NaN) never reach the result.
+Discontinuities are ignored, as in ``Expression.diff``.
+"""
+from . import _expressions as ex
+from ._kernelgen import (DOUBLE_PRECISION, _cname, cell_writer,
+                         forward_sweep, real_literal)
+
+
+class _Rev:
+    """Reverse-mode code emitter for one cell."""
+
+    def __init__(self, cm, precision):
+        self.cm = cm
+        self.p = precision
+        self.w = cell_writer(cm, precision, False)
+        self.bars = {}          # LhsExpression -> C name of its adjoint
+        states = set(cm.states)
+        for s in cm.states:
+            self.bars[ex.Name(s)] = _cname(s, 'bx_')
+            self.bars[ex.Derivative(ex.Name(s))] = _cname(s, 'bd_')
+        for var in cm.order:
+            if var not in states:
+                self.bars[ex.Name(var)] = _cname(var, 'bv_')
+        for var in cm.per_cell_constants:
+            self.bars[ex.Name(var)] = _cname(var, 'bc_')
+        for k, var in enumerate(cm.fields):
+            self.bars[ex.Name(var)] = 'bf' + str(k)
+        for var, what in cm.bound.items():
+            if what == 'idiff':
+                self.bars[ex.Name(var)] = 'b_idiff'
+        self.out = []
+
+    def lit(self, v):
+        return real_literal(v, self.p)
+
+    def v(self, e):
+        """C code of the forward value of sub-expression ``e``."""
+        return self.w.ex(e)
+
+    def mul(self, g, v):
+        """Adjoint ``g`` times forward value ``v``, zero when ``g`` is zero
+        (so an overflowed forward value met by a zero adjoint gives 0, not
+        0 * inf = NaN)."""
+        return 'mk_gmul(' + g + ', ' + v + ')'
+
+    def div(self, a, b):
+        if self.p == DOUBLE_PRECISION:
+            return '(' + a + ' / ' + b + ')'
+        return '(' + a + ' * __builtin_amdgcn_rcpf(' + b + '))'
+
+    def gdiv(self, g, b):
+        """Adjoint ``g`` divided by forward value ``b``, zero when ``g`` is
+        zero."""
+        return 'mk_gdiv(' + g + ', ' + b + ')'
+
+    def acc(self, target, contrib, guard):
+        if guard:
+            self.out.append('    ' + target + ' += (' + ' && '.join(guard)
+                            + ') ? (' + contrib + ') : (Real)0;')
+        else:
+            self.out.append('    ' + target + ' += ' + contrib + ';')
+
+    def rev(self, e, g, guard=()):
+        """Adds d(output)/d(leaves of e) given the adjoint ``g`` of e."""
+        if isinstance(e, ex.Number):
+            return
+        if isinstance(e, (ex.Name, ex.Derivative)):
+            b = self.bars.get(e)
+            if b is not None:
+                self.acc(b, g, guard)
+            return
+        if isinstance(e, ex.PrefixPlus):
+            return self.rev(e[0], g, guard)
+        if isinstance(e, ex.PrefixMinus):
+            return self.rev(e[0], '(-' + g + ')', guard)
+        if isinstance(e, ex.Plus):
+            self.rev(e[0], g, guard)
+            return self.rev(e[1], g, guard)
+        if isinstance(e, ex.Minus):
+            self.rev(e[0], g, guard)
+            return self.rev(e[1], '(-' + g + ')', guard)
+        if isinstance(e, ex.Multiply):
+            a, b = e
+            self.rev(a, self.mul(g, self.v(b)), guard)
+            return self.rev(b, self.mul(g, self.v(a)), guard)
+        if isinstance(e, ex.Divide):
+            a, b = e
+            vb = self.v(b)
+            self.rev(a, self.gdiv(g, vb), guard)
+            # d(a/b)/db = -(a/b)/b
+            q = self.div(self.v(e), vb)
+            return self.rev(b, self.mul('(-' + g + ')', q), guard)
+        if isinstance(e, (ex.Quotient, ex.Floor, ex.Ceil)):
+            return
+        if isinstance(e, ex.Remainder):
+            a, b = e
+            self.rev(a, g, guard)
+            fl = '__builtin_floor' if self.p == DOUBLE_PRECISION else 'floorf'
+            return self.rev(b, self.mul('(-' + g + ')', fl + '('
+                            + self.div(self.v(a), self.v(b)) + ')'), guard)
+        if isinstance(e, ex.Power):
+            a, b = e
+            if isinstance(b, ex.Number):
+                n = b.eval()
+                if n == int(n) and abs(n - 1) <= 64:
+                    pw = self.w.fn('powi') + '(' + self.v(a) + ', ' \
+                        + str(int(n) - 1) + ')'
+                else:
+                    pw = self.w.fn('pow') + '(' + self.v(a) + ', ' \
+                        + self.lit(n - 1) + ')'
+                return self.rev(a, self.mul(g, '(' + self.lit(n) + ' * '
+                                            + pw + ')'), guard)
+            # a^b: d/da = b a^(b-1) = b (a^b) / a; d/db = a^b log(a)
+            ve = self.v(e)
+            self.rev(a, self.mul(g, '(' + self.v(b) + ' * '
+                                 + self.div(ve, self.v(a)) + ')'), guard)
+            return self.rev(b, self.mul(g, '(' + ve + ' * '
+                                        + self.w.fn('log') + '('
+                                        + self.v(a) + '))'), guard)
+        if isinstance(e, ex.Exp):
+            return self.rev(e[0], self.mul(g, self.v(e)), guard)
+        if isinstance(e, ex.Log):
+            if len(e) == 2:
+                return self.rev(ex.Divide(ex.Log(e[0]), ex.Log(e[1])), g,
+                                guard)
+            return self.rev(e[0], self.gdiv(g, self.v(e[0])), guard)
+        if isinstance(e, ex.Log10):
+            return self.rev(e[0], self.gdiv(
+                g, '(' + self.v(e[0]) + ' * ' + self.lit(2.302585092994046)
+                + ')'), guard)
+        if isinstance(e, ex.Sqrt):
+            return self.rev(e[0], self.gdiv(
+                g, '(' + self.lit(2) + ' * ' + self.v(e) + ')'), guard)
+        if isinstance(e, ex.Sin):
+            return self.rev(e[0], self.mul(g, self.w.fn('cos') + '('
+                                           + self.v(e[0]) + ')'), guard)
+        if isinstance(e, ex.Cos):
+            return self.rev(e[0], self.mul('(-' + g + ')', self.w.fn('sin')
+                                           + '(' + self.v(e[0]) + ')'), guard)
+        if isinstance(e, ex.Tan):
+            c = self.w.fn('cos') + '(' + self.v(e[0]) + ')'
+            return self.rev(e[0], self.gdiv(g, '(' + c + ' * ' + c + ')'),
+                            guard)
+        if isinstance(e, (ex.ASin, ex.ACos)):
+            a = self.v(e[0])
+            s = self.mul(g, self.div(self.lit(1), self.w.fn('sqrt') + '('
+                                     + self.lit(1) + ' - ' + a + ' * ' + a
+                                     + ')'))
+            if isinstance(e, ex.ACos):
+                s = '(-' + s + ')'
+            return self.rev(e[0], s, guard)
+        if isinstance(e, ex.ATan):
+            a = self.v(e[0])
+            return self.rev(e[0], self.gdiv(
+                g, '(' + self.lit(1) + ' + ' + a + ' * ' + a + ')'), guard)
+        if isinstance(e, ex.Abs):
+            a = self.v(e[0])
+            return self.rev(e[0], '((' + a + ' >= ' + self.lit(0) + ') ? '
+                            + g + ' : (-' + g + '))', guard)
+        if isinstance(e, ex.If):
+            c = self.v(e[0])
+            self.rev(e[1], g, tuple(guard) + ('(' + c + ')',))
+            return self.rev(e[2], g, tuple(guard) + ('(!' + c + ')',))
+        if isinstance(e, ex.Piecewise):
+            ops = list(e)
+            prev = []
+            for i in range(0, len(ops) - 1, 2):
+                c = self.v(ops[i])
+                self.rev(ops[i + 1], g, tuple(guard) + tuple(prev)
+                         + ('(' + c + ')',))
+                prev.append('(!' + c + ')')
+            return self.rev(ops[-1], g, tuple(guard) + tuple(prev))
+        if isinstance(e, ex.Condition):
+            return
+        raise ValueError('No adjoint rule for ' + type(e).__name__)
+
+
+def generate_vjp_function(cm, precision):
+    """HIP source of ``mk_cell_vjp`` for a CellModel."""
+    r = _Rev(cm, precision)
+    w = r.w
+    tab = '    '
+    lines = [
+        '__device__ __forceinline__ Real mk_gmul(const Real g, const Real v)',
+        '{',
+        tab + 'return g == (Real)0 ? (Real)0 : g * v;',
+        '}',
+        '',
+        '__device__ __forceinline__ Real mk_gdiv(const Real g, const Real b)',
+        '{',
+        tab + 'return g == (Real)0 ? (Real)0 : ' + r.div('g', 'b') + ';',
+        '}',
+        '',
+        '__device__ __forceinline__ void mk_cell_vjp(',
+        tab + 'const Real* __restrict__ y, const Real t, const Real dt,'
+        ' const Real pace, const Real idiff,',
+        tab + 'const Real* __restrict__ fv, const Real* __restrict__ lam,'
+        ' Real* __restrict__ mu,',
+        tab + 'Real* __restrict__ abar, Real* __restrict__ fbar)',
+        '{',
+        tab + '(void)t; (void)dt; (void)pace; (void)idiff; (void)fv;',
+    ]
+    lines.extend(forward_sweep(cm, w, tab))
+    for b in sorted(set(r.bars.values())):
+        lines.append(tab + 'Real ' + b + ' = (Real)0;')
+    # Outputs: the state update of every state
+    expf = w.fn('exp')
+    for i, s in enumerate(cm.states):
+        x = _cname(s, 'x_')
+        if s in cm.rl_states:
+            inf, tau = cm.rl_states[s]
+            vi, vt = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
+            E = 'E_' + str(i)
+            lines.append(tab + 'const Real ' + E + ' = ' + expf + '('
+                         + r.div('-dt', vt) + ');')
+            # y' = inf - (inf - x) E: dy'/dx = E, dy'/dinf = 1 - E,
+            # dy'/dtau = -(inf - x) E dt / tau^2
+            lines.append(tab + _cname(s, 'bx_') + ' += lam[' + str(i)
+                         + '] * ' + E + ';')
+            lines.append(tab + r.bars[ex.Name(inf)] + ' += lam[' + str(i)
+                         + '] * ((Real)1 - ' + E + ');')
+            lines.append(tab + r.bars[ex.Name(tau)] + ' += -lam[' + str(i)
+                         + '] * (' + vi + ' - ' + x + ') * ' + E + ' * '
+                         + r.div('dt', '(' + vt + ' * ' + vt + ')') + ';')
+        else:
+            lines.append(tab + _cname(s, 'bx_') + ' += lam[' + str(i)
+                         + '];')
+            lines.append(tab + _cname(s, 'bd_') + ' += dt * lam[' + str(i)
+                         + '];')
+    # Reverse dependency order
+    for var in reversed(cm.order):
+        if var.is_state():
+            if var in cm.rl_states:
+                continue
+            r.rev(var.rhs(), r.bars[ex.Derivative(ex.Name(var))])
+        else:
+            r.rev(var.rhs(), r.bars[ex.Name(var)])
+    for var in reversed(cm.per_cell_constants):
+        r.rev(var.rhs(), r.bars[ex.Name(var)])
+    lines.extend(r.out)
+    for i, s in enumerate(cm.states):
+        lines.append(tab + 'mu[' + str(i) + '] = ' + _cname(s, 'bx_') + ';')
+    lines.append(tab + '*abar = ' + ('b_idiff' if 'b_idiff' in
+                                     r.bars.values() else '(Real)0') + ';')
+    for k in range(len(cm.fields)):
+        lines.append(tab + 'fbar[' + str(k) + '] = bf' + str(k) + ';')
+    lines.append('}')
+    return '\n'.join(lines) + '\n'

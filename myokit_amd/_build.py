@@ -40,17 +40,19 @@ def build_library(verbose=False):
     return OUT
 
 
-def check_kernel_compiles(precision=32, coupling=1):
-    """Generates the LR1991 kernel and compiles it for gfx950 with hiprtc
-    (no device needed). Returns the code object size in bytes."""
+def check_kernel_compiles(precision=32, coupling=1, adjoint=False, rl=True,
+                          fields=()):
+    """Generates the LR1991 kernel (with the adjoint kernels if ``adjoint``)
+    and compiles it for gfx950 with hiprtc (no device needed)."""
     import ctypes
     from . import _native, _kernelgen
     from .simulation import _prepare_cell_model
     from ._parser import load_model
     model = load_model('example')
-    cm = _prepare_cell_model(model, rl=True, diffusion=True, fields=[],
-                             quantities=['membrane.V'])
-    src = _kernelgen.generate_kernel(cm, precision, coupling)
+    cm = _prepare_cell_model(model, rl=rl, diffusion=True,
+                             fields=list(fields), quantities=['membrane.V'])
+    src = _kernelgen.generate_kernel(cm, precision, coupling,
+                                     adjoint=adjoint)
     prog = ctypes.c_void_p()
     log = ctypes.create_string_buffer(1 << 16)
     _native.check(_native.lib().mkh_program_build(

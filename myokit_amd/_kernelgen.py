@@ -369,12 +369,10 @@ def _cname(var, prefix='v_'):
     return prefix + var.uname()
 
 
-def generate_cell_function(cm, precision, native_maths=False):
-    """Returns the HIP source of ``mk_cell<LOG>`` for a CellModel."""
+def cell_writer(cm, precision, native_maths=False):
+    """A HipExpressionWriter with the C names of one cell's variables."""
     w = HipExpressionWriter(precision, native_maths)
     w.constants = cm.constants
-    lines = []
-    tab = '    '
     bound_names = {'time': 't', 'pace': 'pace', 'idiff': 'idiff'}
     for var, what in cm.bound.items():
         w.names[ex.Name(var)] = bound_names[what]
@@ -388,6 +386,36 @@ def generate_cell_function(cm, precision, native_maths=False):
     for var in cm.order:
         if not var.is_state():
             w.names[ex.Name(var)] = _cname(var)
+    return w
+
+
+def forward_sweep(cm, w, tab='    '):
+    """C statements evaluating every equation of one cell (states read from
+    ``y``), in dependency order."""
+    lines = []
+    for i, s in enumerate(cm.states):
+        lines.append(tab + 'const Real ' + _cname(s, 'x_') + ' = y['
+                     + str(i) + '];')
+    for var in cm.per_cell_constants:
+        lines.append(tab + 'const Real ' + _cname(var, 'c_') + ' = '
+                     + w.ex(var.rhs()) + ';')
+    for var in cm.order:
+        if var.is_state():
+            if var in cm.rl_states:
+                continue
+            lines.append(tab + 'const Real ' + _cname(var, 'd_') + ' = '
+                         + w.ex(var.rhs()) + ';')
+        else:
+            lines.append(tab + 'const Real ' + _cname(var) + ' = '
+                         + w.ex(var.rhs()) + ';')
+    return lines
+
+
+def generate_cell_function(cm, precision, native_maths=False):
+    """Returns the HIP source of ``mk_cell<LOG>`` for a CellModel."""
+    w = cell_writer(cm, precision, native_maths)
+    lines = []
+    tab = '    '
 
     lines.append('template <bool LOG>')
     lines.append('__device__ __forceinline__ void mk_cell(')
@@ -398,22 +426,8 @@ def generate_cell_function(cm, precision, native_maths=False):
     lines.append('{')
     lines.append(tab + '(void)t; (void)dt; (void)pace; (void)idiff;'
                  ' (void)fv; (void)q;')
-    for i, s in enumerate(cm.states):
-        lines.append(tab + 'const Real ' + _cname(s, 'x_') + ' = y['
-                     + str(i) + '];')
-    for var in cm.per_cell_constants:
-        lines.append(tab + 'const Real ' + _cname(var, 'c_') + ' = '
-                     + w.ex(var.rhs()) + ';')
+    lines.extend(forward_sweep(cm, w, tab))
     rl = cm.rl_states
-    for var in cm.order:
-        if var.is_state():
-            if var in rl:
-                continue
-            lines.append(tab + 'const Real ' + _cname(var, 'd_') + ' = '
-                         + w.ex(var.rhs()) + ';')
-        else:
-            lines.append(tab + 'const Real ' + _cname(var) + ' = '
-                         + w.ex(var.rhs()) + ';')
     if cm.quantities:
         lines.append(tab + 'if (LOG) {')
         for k, var in enumerate(cm.quantities):
@@ -500,7 +514,7 @@ def sheet_enabled(precision):
 
 
 def generate_kernel(cm, precision, coupling, block_threads=256,
-                    native_maths=False):
+                    native_maths=False, adjoint=False):
     """Returns the complete HIP source for hiprtc."""
     if block_threads % 64 or not 64 <= block_threads <= 1024:
         raise ValueError('block_threads must be a multiple of 64 in'
@@ -528,6 +542,10 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         generate_cell_function(cm, precision, native_maths),
         _read('mk_drivers.hip'),
     ]
+    if adjoint:
+        from ._adjointgen import generate_vjp_function
+        parts.append(generate_vjp_function(cm, precision))
+        parts.append(_read('mk_adjoint.hip'))
     return '\n'.join(parts)
 
 

@@ -78,6 +78,9 @@ SIGNATURES = {
     'mkh_sim_reset_timing': (_i, [_vp]),
     'mkh_sim_log_rows': (_i, [_vp, _pll]),
     'mkh_sim_driver': (_i, [_vp, _pi]),
+    'mkh_adj_run': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll, _i,
+                         _pd, _pd, _pd]),
+    'mkh_adj_trace': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll]),
     'mkh_pacing_levels': (_i, [_pd, _ll, _d, _pd, _ll, _pd, _pd]),
     'mkh_plan': (_i, [_d, _d, _d, _pd, _ll, _d, _pd, _pd, _pd, _pi, _ll,
                       _pll]),

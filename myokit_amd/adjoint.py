@@ -1,0 +1,168 @@
+"""
+Gradients of a tissue simulation by the discrete adjoint (the reference has
+no gradients; this is an addition on the same simulation object).
+
+``TissueAdjoint(sim, probes)`` takes a configured ``SimulationOpenCL``. The
+parameters are the fields set on it with ``set_field`` and the conductances
+``gx``/``gy`` (for ``set_connections`` graphs: a common scale ``s`` of all
+conductances). For a run of ``duration`` from the simulation's current state
+and time, with one target row per time step,
+
+    L = sum_{s=1..N} sum_p (V(t_s)[probe p] - target[s-1, p])^2,
+
+``loss_and_gradient`` returns ``L``, ``dL/dfield`` (arrays shaped like the
+field values), ``dL/dgx`` and ``dL/dgy``. The adjoint runs on the device:
+forward with a checkpoint every ``checkpoint_every`` steps, then per
+segment a recomputation of the states and the reverse steps. Loss and
+gradients accumulate in double precision whatever the simulation precision.
+
+``TissueLoss`` wraps this as a ``torch.autograd.Function`` so that the loss
+can sit inside a PyTorch optimisation loop.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._err import SimultaneousProtocolEventError
+
+
+class TissueAdjoint:
+    """Discrete-adjoint gradients for a configured SimulationOpenCL."""
+
+    def __init__(self, sim, probes, checkpoint_every=200):
+        self.sim = sim
+        p = np.asarray(probes)
+        if p.ndim == 2:
+            # (x, y) pairs in 2D
+            p = p[:, 0] + p[:, 1] * sim._nx
+        self.probes = np.ascontiguousarray(p.reshape(-1), dtype=np.int64)
+        if np.any(self.probes < 0) or np.any(self.probes >= sim._ntotal):
+            raise ValueError('Probe cell out of range.')
+        self.checkpoint_every = int(checkpoint_every)
+        if self.checkpoint_every < 1:
+            raise ValueError('checkpoint_every must be at least 1.')
+        if sim._connections is not None:
+            # Adjoint kernels use the global connection layout
+            sim.set_engine_option('driver', _native.DRIVER_GRID)
+
+    def _events(self):
+        sim = self.sim
+        ev = sim._protocol.to_arrays().reshape(-1) if sim._protocol else \
+            np.zeros(0)
+        ev = np.ascontiguousarray(ev if len(ev) else np.zeros(1))
+        nev = len(sim._protocol.events()) if sim._protocol else 0
+        return ev, nev
+
+    def n_steps(self, duration):
+        """Number of time steps of a run of ``duration`` from the current
+        time (the length the target must have)."""
+        sim = self.sim
+        ev, nev = self._events()
+        n = ctypes.c_longlong(0)
+        _native.check(_native.lib().mkh_plan(
+            sim._time, sim._time + duration, sim._step_size,
+            _native.ptr(ev, ctypes.c_double), nev, 1e300, None, None, None,
+            None, 1 << 62, ctypes.byref(n)), 'mkh_plan')
+        return n.value
+
+    def _engine(self):
+        sim = self.sim
+        eng, cm = sim._get_engine([], adjoint=True)
+        sim._configure(eng)
+        return eng
+
+    def _check(self, code, what):
+        if code == _native.MKH_E_PACING_SIMULTANEOUS:
+            raise SimultaneousProtocolEventError(_native.last_error())
+        _native.check(code, what)
+
+    def loss_and_gradient(self, duration, target):
+        """Returns ``(loss, field_gradients, dgx, dgy)``; ``target`` has
+        shape ``(n_steps(duration), n_probes)``."""
+        sim = self.sim
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        if target.ndim != 2 or target.shape[1] != len(self.probes):
+            raise ValueError('Target must have shape (steps, n_probes).')
+        eng = self._engine()
+        ev, nev = self._events()
+        nf = len(sim._fields)
+        gf = np.zeros(max(nf, 1) * sim._ntotal)
+        gg = np.zeros(2)
+        loss = ctypes.c_double(0)
+        self._check(_native.lib().mkh_adj_run(
+            eng.h, sim._time, sim._time + duration, sim._step_size,
+            _native.ptr(ev, ctypes.c_double), nev,
+            _native.ptr(self.probes, ctypes.c_longlong), len(self.probes),
+            _native.ptr(target, ctypes.c_double), target.shape[0],
+            self.checkpoint_every, ctypes.byref(loss),
+            _native.ptr(gf, ctypes.c_double),
+            _native.ptr(gg, ctypes.c_double)), 'mkh_adj_run')
+        shape = (sim._ny, sim._nx) if len(sim._dims) == 2 else (sim._nx,)
+        grads = {}
+        for k, var in enumerate(sim._fields.keys()):
+            grads[var.qname()] = gf[k * sim._ntotal:(k + 1) * sim._ntotal] \
+                .reshape(shape).copy()
+        return loss.value, grads, gg[0], gg[1]
+
+    def trace(self, duration):
+        """V at the probes after every time step of a run of ``duration``
+        from the current state: shape ``(n_steps, n_probes)``, the quantity
+        the loss compares with the target. Leaves the simulation's state and
+        time unchanged."""
+        sim = self.sim
+        eng = self._engine()
+        ev, nev = self._events()
+        n = self.n_steps(duration)
+        out = np.zeros((n, len(self.probes)))
+        self._check(_native.lib().mkh_adj_trace(
+            eng.h, sim._time, sim._time + duration, sim._step_size,
+            _native.ptr(ev, ctypes.c_double), nev,
+            _native.ptr(self.probes, ctypes.c_longlong), len(self.probes),
+            _native.ptr(out, ctypes.c_double), n), 'mkh_adj_trace')
+        return out
+
+
+try:
+    import torch
+
+    class TissueLoss(torch.autograd.Function):
+        """``loss = TissueLoss.apply(adjoint, duration, target, gx, gy,
+        *field_tensors)`` with gradients for gx, gy and every field tensor
+        (shaped like the values given to ``set_field``), by the discrete
+        adjoint. Field tensors replace the simulation's fields, in the order
+        of ``adjoint.sim._fields``."""
+
+        @staticmethod
+        def forward(ctx, adj, duration, target, gx, gy, *fields):
+            sim = adj.sim
+            if sim._diffusion_enabled and sim._connections is None and \
+                    sim._gx_field is None:
+                sim.set_conductance(float(gx), float(gy))
+            names = [v.qname() for v in sim._fields.keys()]
+            if len(fields) != len(names):
+                raise ValueError('One tensor per field is required.')
+            for name, f in zip(names, fields):
+                sim.set_field(name, f.detach().cpu().numpy().astype(float))
+            t = target.detach().cpu().numpy() if hasattr(target, 'detach') \
+                else np.asarray(target)
+            loss, grads, dgx, dgy = adj.loss_and_gradient(duration, t)
+            ctx.dgx = dgx
+            ctx.dgy = dgy
+            ctx.grads = [torch.as_tensor(grads[n], dtype=f.dtype,
+                                         device=f.device)
+                         for n, f in zip(names, fields)]
+            dev = fields[0].device if fields else 'cpu'
+            return torch.tensor(loss, dtype=torch.float64, device=dev)
+
+        @staticmethod
+        def backward(ctx, gout):
+            g = float(gout)
+            out = [None, None, None,
+                   torch.tensor(g * ctx.dgx, dtype=torch.float64),
+                   torch.tensor(g * ctx.dgy, dtype=torch.float64)]
+            out.extend(g * x for x in ctx.grads)
+            return tuple(out)
+
+except ImportError:     # pragma: no cover
+    TissueLoss = None

@@ -47,6 +47,26 @@ struct MkArgsT {
     R gx, gy;                   /* homogeneous conductances */
 };
 
+/* Discrete adjoint of one step (see mk_adjoint.hip). */
+template <typename R>
+struct MkAdjT {
+    const R* yn;            /* state at t_s (SoA) */
+    const R* lam;           /* adjoint of the state at t_{s+1} (SoA) */
+    R* mu;                  /* (dF/dy)^T lam per cell (SoA); NULL = zero */
+    R* abar;                /* adjoint of each cell's diffusion current */
+    R* lam_out;             /* adjoint of the state at t_s (SoA) */
+    double* gfield;         /* [n_field][npad] field gradients (accumulated) */
+    double* gx_acc;         /* [npad] d/dgx contributions (accumulated) */
+    double* gy_acc;         /* [npad] d/dgy contributions (accumulated) */
+    double* loss_acc;       /* [n_probe] squared errors (accumulated) */
+    const int* probe_of;    /* [npad] probe index of each cell, or -1 */
+    const double* target;   /* [rows][n_probe] target potentials */
+    const long long* probes;/* [n_probe] probe cells */
+    double* trace;          /* [rows][n_probe] V at the probes, or NULL */
+    int n_probe;
+    int target_row;         /* target row for the loss at t_s, or -1 */
+};
+
 /* Persistent sheet driver: neighbour exchange between workgroup tiles. */
 template <typename R>
 struct MkSheetT {

@@ -60,6 +60,8 @@ struct Program {
     hipModule_t module = nullptr;
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
     hipFunction_t f_sheet = nullptr;
+    hipFunction_t f_adj_vjp = nullptr, f_adj_couple = nullptr,
+                  f_adj_probe = nullptr;
     std::vector<char> code;
 };
 
@@ -210,6 +212,9 @@ struct SimBase {
     virtual int reset_timing() = 0;
     virtual long long log_rows() = 0;
     virtual int get_driver() = 0;
+    virtual int adj_run(double, double, double, const double*, long long,
+                        const long long*, long long, const double*, long long,
+                        int, double*, double*, double*, double*) = 0;
 };
 
 template <typename R>
@@ -900,10 +905,12 @@ struct Sim : SimBase {
         } else if (k == "driver") {
             forced_driver = (int)v;
             if (have_graph) {
-                driver = (int)v;
-            } else {
-                choose_driver();
+                std::vector<long long> i(ci), jj(cj);
+                std::vector<double> g(cg);
+                return set_connections(i.data(), jj.data(), g.data(),
+                                       (long long)i.size());
             }
+            choose_driver();
         } else {
             return fail(MKH_E_ARG, "unknown option " + k);
         }
@@ -923,6 +930,233 @@ struct Sim : SimBase {
         kernel_ms = 0;
         launches = 0;
         cell_steps = 0;
+        return 0;
+    }
+
+
+    unsigned grid_launch_size()
+    {
+        const unsigned bt = (unsigned)cfg.block_threads;
+        if ((cfg.coupling == MKH_COUPLING_STENCIL ||
+             cfg.coupling == MKH_COUPLING_HETERO) && cfg.ny > 1) {
+            long long tx = (cfg.nx + 63) / 64;
+            long long ty = (cfg.ny + (bt / 64) - 1) / (bt / 64);
+            return (unsigned)(tx * ty);
+        }
+        return (unsigned)((n + bt - 1) / bt);
+    }
+
+    /* Discrete adjoint of a run from the current state (which it leaves
+     * unchanged). Forward with a checkpoint every K steps, then per segment
+     * (last first): recompute the segment's states into a trajectory buffer
+     * and take the reverse steps. */
+    int adj_run(double tmin, double tmax, double dt, const double* events,
+                long long nev, const long long* probes, long long nprobe,
+                const double* target, long long nrows, int K, double* loss,
+                double* gfield_out, double* gg_out, double* trace_out) override
+    {
+        /* trace_out != NULL: forward only, V at the probes after each step
+         * into trace_out[s * nprobe + p] (nrows = its row capacity) */
+        if (!prog->f_adj_vjp)
+            return fail(MKH_E_STATE, "kernel was generated without the "
+                                     "adjoint");
+        if (cfg.coupling == MKH_COUPLING_GRAPH && driver != MKH_DRIVER_GRID)
+            return fail(MKH_E_STATE, "graph adjoint needs the grid layout");
+        if (K < 1) return fail(MKH_E_ARG, "checkpoint interval must be >= 1");
+        Planner pl;
+        int f = pl.init(tmin, tmax, dt, events, nev, HUGE_VAL);
+        if (f) return pacing_error(f);
+        std::vector<Step> all;
+        StepRec r;
+        while (!pl.done) {
+            f = pl.next(r);
+            if (f) return pacing_error(f);
+            Step st;
+            st.t = (R)r.t;
+            st.dt = (R)r.dt;
+            st.pace = (R)r.pace;
+            st.row = -1;
+            all.push_back(st);
+        }
+        const long long N = (long long)all.size();
+        if (N == 0) return fail(MKH_E_ARG, "empty run");
+        if (nrows < N)
+            return fail(MKH_E_ARG, std::string(trace_out ? "trace" : "target")
+                                       + " needs one row per time step ("
+                                       + std::to_string(N) + ")");
+        for (long long k = 0; k < nprobe; ++k)
+            if (probes[k] < 0 || probes[k] >= n)
+                return fail(MKH_E_ARG, "probe cell out of range");
+        const size_t SN = (size_t)cfg.n_state * npad;
+        if (trace_out) K = (int)std::min<long long>(N, 1 << 30);
+        const long long nseg = (N + K - 1) / K;
+        DevBuf<Step> dsteps;
+        DevBuf<R> ckpt, traj, lamA, lamB, mu, abar;
+        DevBuf<double> gfield, gxa, gya, lossa, dtarget;
+        DevBuf<int> probe_of;
+        DevBuf<long long> dprobes;
+        DevBuf<double> dtrace;
+        MKH_HIP(dsteps.alloc(N));
+        MKH_HIP(hipMemcpy(dsteps.p, all.data(), N * sizeof(Step),
+                          hipMemcpyHostToDevice));
+        MKH_HIP(ckpt.alloc(trace_out ? SN : nseg * SN));
+        MKH_HIP(traj.alloc((size_t)(trace_out ? 2 : K + 1) * SN));
+        MKH_HIP(lamA.alloc(SN));
+        MKH_HIP(lamB.alloc(SN));
+        MKH_HIP(mu.alloc(SN));
+        MKH_HIP(abar.alloc(npad));
+        MKH_HIP(gfield.alloc(std::max<size_t>(1, (size_t)cfg.n_field * npad)));
+        MKH_HIP(gxa.alloc(npad));
+        MKH_HIP(gya.alloc(npad));
+        MKH_HIP(lossa.alloc(std::max<long long>(1, nprobe)));
+        MKH_HIP(hipMemset(gfield.p, 0, gfield.n * sizeof(double)));
+        MKH_HIP(hipMemset(gxa.p, 0, npad * sizeof(double)));
+        MKH_HIP(hipMemset(gya.p, 0, npad * sizeof(double)));
+        MKH_HIP(hipMemset(lossa.p, 0, lossa.n * sizeof(double)));
+        MKH_HIP(hipMemset(abar.p, 0, npad * sizeof(R)));
+        std::vector<int> pof(npad, -1);
+        for (long long k = 0; k < nprobe; ++k) pof[probes[k]] = (int)k;
+        MKH_HIP(probe_of.alloc(npad));
+        MKH_HIP(hipMemcpy(probe_of.p, pof.data(), npad * sizeof(int),
+                          hipMemcpyHostToDevice));
+        MKH_HIP(dprobes.alloc(std::max<long long>(1, nprobe)));
+        if (nprobe > 0)
+            MKH_HIP(hipMemcpy(dprobes.p, probes, nprobe * sizeof(long long),
+                              hipMemcpyHostToDevice));
+        if (trace_out) MKH_HIP(dtrace.alloc(std::max<long long>(1, N * nprobe)));
+        MKH_HIP(dtarget.alloc(std::max<long long>(1, N * nprobe)));
+        if (nprobe > 0 && !trace_out)
+            MKH_HIP(hipMemcpy(dtarget.p, target, N * nprobe * sizeof(double),
+                              hipMemcpyHostToDevice));
+        int m1 = -1;
+        MKH_HIP(hipMemcpy(halt.p, &m1, sizeof(int), hipMemcpyHostToDevice));
+
+        Args a = base_args();
+        a.steps = dsteps.p;
+        const unsigned bt = (unsigned)cfg.block_threads;
+        const unsigned ggrid = grid_launch_size();
+        const unsigned cgrid = (unsigned)((n + bt - 1) / bt);
+        auto grid_step = [&](R* src, R* dst, int si) -> int {
+            void* params[] = {&a, &src, &dst, &si};
+            MKH_HIP(hipModuleLaunchKernel(prog->f_grid, ggrid, 1, 1, bt, 1,
+                                          1, 0, stream, params, nullptr));
+            return 0;
+        };
+        MkAdjT<R> j;
+        std::memset(&j, 0, sizeof(j));
+        j.gfield = gfield.p;
+        j.gx_acc = gxa.p;
+        j.gy_acc = gya.p;
+        j.loss_acc = lossa.p;
+        j.probe_of = probe_of.p;
+        j.target = dtarget.p;
+        j.probes = dprobes.p;
+        j.trace = dtrace.p;
+        j.n_probe = (int)nprobe;
+        j.abar = abar.p;
+        const unsigned pgrid = (unsigned)((nprobe + 63) / 64);
+        /* Forward: two work buffers from the trajectory storage */
+        R* wa = traj.p;
+        R* wb = traj.p + SN;
+        MKH_HIP(hipMemcpyAsync(wa, state[cur].p, SN * sizeof(R),
+                               hipMemcpyDeviceToDevice, stream));
+        for (long long sg = 0; sg < nseg; ++sg) {
+            if (!trace_out)
+                MKH_HIP(hipMemcpyAsync(ckpt.p + sg * SN, wa, SN * sizeof(R),
+                                       hipMemcpyDeviceToDevice, stream));
+            const long long s0 = sg * K, s1 = std::min(N, s0 + K);
+            for (long long si = s0; si < s1; ++si) {
+                int e = grid_step(wa, wb, (int)si);
+                if (e) return e;
+                std::swap(wa, wb);
+                if (trace_out && nprobe > 0) {
+                    MkAdjT<R> jt = j;
+                    jt.yn = wa;
+                    int sii = (int)si;
+                    void* params[] = {&a, &jt, &sii};
+                    MKH_HIP(hipModuleLaunchKernel(prog->f_adj_probe, pgrid, 1,
+                                                  1, 64, 1, 1, 0, stream,
+                                                  params, nullptr));
+                }
+            }
+        }
+        if (trace_out) {
+            MKH_HIP(hipStreamSynchronize(stream));
+            if (nprobe > 0)
+                MKH_HIP(hipMemcpy(trace_out, dtrace.p,
+                                  N * nprobe * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+            return 0;
+        }
+        /* lam_N = dL/dy_N */
+        R* lam = lamA.p;
+        R* lam2 = lamB.p;
+        {
+            MkAdjT<R> j0 = j;
+            j0.yn = wa;
+            j0.mu = nullptr;
+            j0.lam_out = lam;
+            j0.target_row = (int)(N - 1);
+            int si = (int)N;
+            void* params[] = {&a, &j0, &si};
+            MKH_HIP(hipModuleLaunchKernel(prog->f_adj_couple, cgrid, 1, 1,
+                                          bt, 1, 1, 0, stream, params,
+                                          nullptr));
+        }
+        /* Backward over segments */
+        for (long long sg = nseg - 1; sg >= 0; --sg) {
+            const long long s0 = sg * K, s1 = std::min(N, s0 + K);
+            MKH_HIP(hipMemcpyAsync(traj.p, ckpt.p + sg * SN, SN * sizeof(R),
+                                   hipMemcpyDeviceToDevice, stream));
+            for (long long si = s0; si + 1 < s1; ++si) {
+                int e = grid_step(traj.p + (si - s0) * SN,
+                                  traj.p + (si - s0 + 1) * SN, (int)si);
+                if (e) return e;
+            }
+            for (long long si = s1 - 1; si >= s0; --si) {
+                MkAdjT<R> js = j;
+                js.yn = traj.p + (si - s0) * SN;
+                js.lam = lam;
+                js.mu = mu.p;
+                js.lam_out = lam2;
+                js.target_row = si >= 1 ? (int)(si - 1) : -1;
+                int sii = (int)si;
+                void* params[] = {&a, &js, &sii};
+                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_vjp, cgrid, 1, 1,
+                                              bt, 1, 1, 0, stream, params,
+                                              nullptr));
+                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_couple, cgrid, 1,
+                                              1, bt, 1, 1, 0, stream, params,
+                                              nullptr));
+                std::swap(lam, lam2);
+            }
+        }
+        MKH_HIP(hipStreamSynchronize(stream));
+        /* Reductions in double precision, in cell order */
+        std::vector<double> h(std::max<size_t>(npad, gfield.n));
+        if (gfield_out && cfg.n_field > 0) {
+            MKH_HIP(hipMemcpy(h.data(), gfield.p, gfield.n * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            for (int k = 0; k < cfg.n_field; ++k)
+                for (long long c = 0; c < n; ++c)
+                    gfield_out[k * n + c] = h[(size_t)k * npad + c];
+        }
+        double sx = 0, sy = 0, sl = 0;
+        MKH_HIP(hipMemcpy(h.data(), gxa.p, npad * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        for (long long c = 0; c < n; ++c) sx += h[c];
+        MKH_HIP(hipMemcpy(h.data(), gya.p, npad * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        for (long long c = 0; c < n; ++c) sy += h[c];
+        std::vector<double> hl(lossa.n);
+        MKH_HIP(hipMemcpy(hl.data(), lossa.p, lossa.n * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        for (long long k = 0; k < nprobe; ++k) sl += hl[k];
+        if (gg_out) {
+            gg_out[0] = sx;
+            gg_out[1] = sy;
+        }
+        if (loss) *loss = sl;
         return 0;
     }
 
@@ -1043,6 +1277,17 @@ int mkh_program_build(const char* source, const char* options,
         MKH_HIP(hipModuleGetFunction(&prog->f_block, prog->module, "mk_block"));
         MKH_HIP(hipModuleGetFunction(&prog->f_grid, prog->module, "mk_grid"));
         MKH_HIP(hipModuleGetFunction(&prog->f_sheet, prog->module, "mk_sheet"));
+        /* present only in kernels generated with gradients */
+        if (hipModuleGetFunction(&prog->f_adj_vjp, prog->module,
+                                 "mk_adj_vjp") != hipSuccess ||
+            hipModuleGetFunction(&prog->f_adj_couple, prog->module,
+                                 "mk_adj_couple") != hipSuccess ||
+            hipModuleGetFunction(&prog->f_adj_probe, prog->module,
+                                 "mk_adj_probe") != hipSuccess) {
+            prog->f_adj_vjp = prog->f_adj_couple = prog->f_adj_probe =
+                nullptr;
+            (void)hipGetLastError();
+        }
     }
     *program = prog.release();
     return 0;
@@ -1221,6 +1466,29 @@ int mkh_sim_reset_timing(void* sim)
 {
     CHECK_SIM();
     return SIM->reset_timing();
+}
+
+int mkh_adj_run(void* sim, double tmin, double tmax, double dt,
+                const double* events, long long n_events,
+                const long long* probes, long long n_probes,
+                const double* target, long long n_target_rows,
+                int checkpoint_every, double* loss, double* grad_fields,
+                double* grad_g)
+{
+    CHECK_SIM();
+    return SIM->adj_run(tmin, tmax, dt, events, n_events, probes, n_probes,
+                        target, n_target_rows, checkpoint_every, loss,
+                        grad_fields, grad_g, nullptr);
+}
+
+int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,
+                  const double* events, long long n_events,
+                  const long long* probes, long long n_probes, double* trace,
+                  long long n_rows)
+{
+    if (!sim || !trace) return fail(MKH_E_ARG, "null argument");
+    return SIM->adj_run(tmin, tmax, dt, events, n_events, probes, n_probes,
+                        nullptr, n_rows, 1, nullptr, nullptr, nullptr, trace);
 }
 
 int mkh_sim_driver(void* sim, int* driver)

@@ -655,14 +655,14 @@ class SimulationOpenCL:
         return {'kernel_ms': ms.value, 'launches': nl.value,
                 'cell_steps': cs.value}
 
-    def _get_engine(self, quantities):
+    def _get_engine(self, quantities, adjoint=False):
         fields = list(self._fields.keys())
         cm = _kernelgen.CellModel(self._model, self._bound_variables,
                                   self._rl_states, fields, quantities)
         coupling = self._coupling()
         src = _kernelgen.generate_kernel(
             cm, self._precision, coupling, self._block_threads,
-            self._native_math)
+            self._native_math, adjoint=adjoint)
         opts = _kernelgen.compile_options(self._precision)
         key = (src, opts, self._ntotal, self._dims)
         if self._engine is not None and self._engine_key == key:

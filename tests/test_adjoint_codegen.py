@@ -1,0 +1,233 @@
+"""
+CPU checks of the generated cell VJP (``mk_cell_vjp``, myokit_amd/_adjointgen):
+the generated HIP function is compiled as C with gcc and compared with central
+finite differences of the oracle's one-cell update (``oracle_cell``, the
+forward-Euler / Rush-Larsen step of ``_sim/openclsim.cl:298-367``):
+
+    mu = (dF/dy)^T lam,  abar = (dF/didiff)^T lam,  fbar = (dF/dfield)^T lam.
+
+The single-precision variant runs with host stand-ins for the two AMDGCN
+builtins it uses (exp2 and reciprocal), and is checked against the double
+variant, including at rest where LR1991's ``ina.a`` switch overflows exp in
+single precision.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import myokit_amd as myokit
+from myokit_amd import _kernelgen
+from myokit_amd._adjointgen import generate_vjp_function
+from myokit_amd.simulation import _prepare_model
+
+from oracle import cgen
+
+CSRC = os.path.join(os.path.dirname(os.path.dirname(
+    os.path.abspath(__file__))), 'myokit_amd', 'csrc')
+
+PRELUDE = '''
+#include <math.h>
+#include <stdbool.h>
+#include "mk_math.h"
+#define __device__ static
+#define __forceinline__ inline
+#define __restrict__ restrict
+#define __builtin_amdgcn_exp2f(x) exp2f(x)
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
+typedef %s Real;
+'''
+
+WRAPPER = '''
+void vjp(const Real* y, Real t, Real dt, Real pace, Real idiff,
+         const Real* fv, const Real* lam, Real* mu, Real* abar, Real* fbar)
+{
+    mk_cell_vjp(y, t, dt, pace, idiff, fv, lam, mu, abar, fbar);
+}
+'''
+
+
+def _gcc(src, path):
+    c = str(path) + '.c'
+    with open(c, 'w') as f:
+        f.write(src)
+    so = str(path) + '.so'
+    r = subprocess.run(['gcc', '-O1', '-fPIC', '-shared', '-std=gnu11',
+                        '-ffp-contract=off', '-I', CSRC, '-o', so, c, '-lm'],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return ctypes.CDLL(so)
+
+
+class Cell:
+    """A prepared model with its oracle step and compiled VJPs."""
+
+    def __init__(self, tmp_path, model, rl, fields, diffusion=True):
+        m, vm, rl_states, bound = _prepare_model(model, rl, diffusion)
+        self.fields = [m.get(f) for f in fields]
+        self.cm = _kernelgen.CellModel(m, bound, rl_states, self.fields, [])
+        self.ns = len(self.cm.states)
+        self.nf = len(self.fields)
+        self.y0 = np.array([s.initial_value(as_float=True)
+                            for s in self.cm.states])
+        src = cgen.model_source(m, bound, rl_states, self.fields, [])
+        self.orc = _gcc(src, tmp_path / 'orc')
+        self.orc.oracle_cell.restype = None
+        d = ctypes.POINTER(ctypes.c_double)
+        self.orc.oracle_cell.argtypes = [
+            d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            ctypes.c_double, d, d, ctypes.c_int]
+        self.vjps = {}
+        for prec, ct, rt in ((64, 'double', ctypes.c_double),
+                             (32, 'float', ctypes.c_float)):
+            src = PRELUDE % ct
+            if prec == 32:
+                src += _kernelgen._FLOAT_HELPERS
+            src += generate_vjp_function(self.cm, prec) + WRAPPER
+            lib = _gcc(src, tmp_path / ('vjp%d' % prec))
+            p = ctypes.POINTER(rt)
+            lib.vjp.restype = None
+            lib.vjp.argtypes = [p, rt, rt, rt, rt, p, p, p, p, p]
+            self.vjps[prec] = (lib, rt)
+
+    def step(self, y, t, dt, pace, idiff, fv):
+        y = np.array(y, dtype=np.float64)
+        fv = np.ascontiguousarray(fv, dtype=np.float64)
+        q = np.zeros(1)
+        d = ctypes.POINTER(ctypes.c_double)
+        self.orc.oracle_cell(y.ctypes.data_as(d), t, dt, pace, idiff,
+                             fv.ctypes.data_as(d), q.ctypes.data_as(d), 0)
+        return y
+
+    def vjp(self, prec, y, t, dt, pace, idiff, fv, lam):
+        lib, rt = self.vjps[prec]
+        npt = np.float64 if prec == 64 else np.float32
+        p = ctypes.POINTER(rt)
+        y = np.ascontiguousarray(y, dtype=npt)
+        fv = np.ascontiguousarray(fv if len(fv) else [0], dtype=npt)
+        lam = np.ascontiguousarray(lam, dtype=npt)
+        mu = np.zeros(self.ns, dtype=npt)
+        ab = np.zeros(1, dtype=npt)
+        fb = np.zeros(max(1, self.nf), dtype=npt)
+        lib.vjp(y.ctypes.data_as(p), t, dt, pace, idiff, fv.ctypes.data_as(p),
+                lam.ctypes.data_as(p), mu.ctypes.data_as(p),
+                ab.ctypes.data_as(p), fb.ctypes.data_as(p))
+        return mu.astype(float), float(ab[0]), fb[:self.nf].astype(float)
+
+    def fd(self, y, t, dt, pace, idiff, fv, lam, rel=1e-4):
+        """Fourth-order central differences of lam . F around (y, idiff,
+        fv): (8 (phi(h) - phi(-h)) - (phi(2h) - phi(-2h))) / 12h."""
+        def phi(yy, ii, ff):
+            return float(np.dot(lam, self.step(yy, t, dt, pace, ii, ff)))
+
+        def d(f, h):
+            return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+        def shifted(v, k, e):
+            v = np.array(v, dtype=float)
+            v[k] += e
+            return v
+        hy = np.array([rel * max(abs(y[k]), 1e-2) for k in range(self.ns)])
+        hi = rel * max(abs(idiff), 1.0)
+        # Fields enter through small currents: a larger step keeps their
+        # effect on V (~1e-11 for 1e-4) far above V's rounding (~1e-14)
+        hf = np.array([1e-2 * max(abs(fv[k]), 1e-2) for k in range(self.nf)])
+        mu = np.array([d(lambda e: phi(shifted(y, k, e), idiff, fv), hy[k])
+                       for k in range(self.ns)])
+        ab = d(lambda e: phi(y, idiff + e, fv), hi)
+        fb = np.array([d(lambda e: phi(y, idiff, shifted(fv, k, e)), hf[k])
+                       for k in range(self.nf)])
+        # Rounding floor of each difference: phi carries an error of about
+        # 2 eps sum |lam_i F_i|, which the formula scales by 1.5 / h
+        err = 2 * 2.3e-16 * float(np.dot(np.abs(lam), np.abs(
+            self.step(y, t, dt, pace, idiff, fv))))
+        self.noise = (1.5 * err / hy, 1.5 * err / hi, 1.5 * err / hf)
+        return mu, ab, fb
+
+
+def _states(cell):
+    """Resting state, a depolarised state and a mid-upstroke state."""
+    y0 = cell.y0
+    iv = [s.qname() for s in cell.cm.states].index('membrane.V')
+    ys = [y0.copy()]
+    y = y0.copy()
+    y[iv] = 20.0
+    ys.append(y)
+    y = y0.copy()
+    y[iv] = -40.5
+    ys.append(y)
+    return ys
+
+
+def _close(a, b, scale, rtol):
+    return np.all(np.abs(a - b) <= rtol * (np.abs(b) + scale))
+
+
+@pytest.mark.parametrize('rl', [False, True])
+def test_vjp_matches_finite_differences_double(tmp_path, rl):
+    cell = Cell(tmp_path, myokit.load_model('example'), rl,
+                ['ina.gNa', 'ica.gCa'])
+    rng = np.random.default_rng(3)
+    fv = np.array([16.0 * 1.1, 0.09 * 0.9])
+    for y in _states(cell):
+        for pace, idiff in ((0.0, 0.0), (1.0, -3.0)):
+            lam = rng.normal(size=cell.ns)
+            mu, ab, fb = cell.vjp(64, y, 1.0, 0.01, pace, idiff, fv, lam)
+            fmu, fab, ffb = cell.fd(y, 1.0, 0.01, pace, idiff, fv, lam)
+            nmu, nab, nfb = cell.noise
+            assert np.all(np.abs(mu - fmu) <= 1e-6 * np.abs(fmu) + nmu), \
+                (mu, fmu, nmu)
+            assert abs(ab - fab) <= 1e-6 * abs(fab) + nab
+            assert np.all(np.abs(fb - ffb) <= 1e-6 * np.abs(ffb) + nfb), \
+                (fb, ffb, nfb)
+
+
+def test_vjp_single_precision_no_overflow_nan(tmp_path):
+    # At rest (V = -84 mV), exp(-(V + 40) / 0.24) = exp(183) overflows in
+    # single precision inside ina.a = 1 - 1 / (1 + exp(...)): the adjoint
+    # reaching that exp is exactly zero and must stay zero, not 0 * inf
+    cell = Cell(tmp_path, myokit.load_model('example'), True,
+                ['ina.gNa', 'ica.gCa'])
+    rng = np.random.default_rng(5)
+    fv = np.array([16.0, 0.09])
+    for y in _states(cell):
+        lam = rng.normal(size=cell.ns)
+        m32, a32, f32 = cell.vjp(32, y, 1.0, 0.01, 0.0, 0.5, fv, lam)
+        m64, a64, f64 = cell.vjp(64, y, 1.0, 0.01, 0.0, 0.5, fv, lam)
+        assert np.all(np.isfinite(m32)) and np.isfinite(a32)
+        assert np.all(np.isfinite(f32))
+        assert _close(m32, m64, np.max(np.abs(m64)), 2e-3)
+        assert a32 == pytest.approx(a64, rel=2e-3)
+        assert _close(f32, f64, np.max(np.abs(f64)), 2e-3)
+
+
+def test_vjp_guards_untaken_branches(tmp_path):
+    # The untaken branch divides by zero at x = 0 (inf); its contribution
+    # is guarded, so the adjoint is that of the taken branch only. sqrt,
+    # log, power and trigonometric rules are exercised on the other state.
+    src = '''[[model]]
+c.x = 0
+c.z = 0.5
+[c]
+t = 0 bind time
+dot(x) = if(x == 0, 2 * z, 1 / x) + piecewise(z < 0, 1 / (x * z), z > 1, 3, z^2)
+dot(z) = sqrt(z) + log(1 + z) * sin(z) + cos(x) * atan(z) + z^2.5 + exp(-z) / (1 + z)
+'''
+    m = myokit.parse_model(src)
+    cell = Cell(tmp_path, m, False, [], diffusion=False)
+    y = np.array([0.0, 0.5])
+    lam = np.array([0.7, -1.3])
+    for prec in (64, 32):
+        mu, ab, _ = cell.vjp(prec, y, 0.0, 0.1, 0.0, 0.0, [], lam)
+        assert np.all(np.isfinite(mu))
+        # F = y + dt f(y); df_x/dz = 2 + 2 z at x = 0
+        dz = lam[1] * 0.1 * (0.5 / np.sqrt(0.5) + np.log(1.5) * np.cos(0.5)
+                             + np.sin(0.5) / 1.5 + 1.0 / (1 + 0.25)
+                             + 2.5 * 0.5 ** 1.5
+                             + (-np.exp(-0.5) * 1.5 - np.exp(-0.5)) / 2.25)
+        exp_z = lam[1] + lam[0] * 0.1 * (2 + 2 * 0.5) + dz
+        assert mu[1] == pytest.approx(exp_z, rel=1e-5 if prec == 32 else 1e-12)
+        # d/dx: the taken branch 2 z has none; cos(x) atan(z) gives -sin(0)
+        assert mu[0] == pytest.approx(lam[0], rel=1e-6)

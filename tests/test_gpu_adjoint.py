@@ -1,0 +1,230 @@
+"""
+GPU tests of the discrete adjoint (myokit_amd.adjoint): gradients against
+central finite differences of the same discrete loss (h = 1e-6, relative),
+against a closed form for a linear model, single against double precision,
+and the torch.autograd bridge.
+"""
+import numpy as np
+import pytest
+
+import myokit_amd as myokit
+from myokit_amd.adjoint import TissueAdjoint
+
+pytestmark = pytest.mark.gpu
+
+
+def cable(n=30, precision=64, rl=False, dt=0.005, seed=1):
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=n, precision=precision, rl=rl)
+    s.set_step_size(dt)
+    s.set_conductance(9.5)
+    rng = np.random.default_rng(seed)
+    s.set_field('ina.gNa', 16 * rng.uniform(0.8, 1.2, n))
+    s.set_field('ica.gCa', 0.09 * rng.uniform(0.8, 1.2, n))
+    return s
+
+
+def fd(f, x, h):
+    return (f(x + h) - f(x - h)) / (2 * h)
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def test_adjoint_cable_fd_fp64():
+    s = cable()
+    probes = [3, 10, 25]
+    adj = TissueAdjoint(s, probes, checkpoint_every=64)
+    dur = 4.0
+    # Target: the trace of a hidden parameter set
+    hidden = cable(seed=2)
+    hidden.set_conductance(11.0)
+    target = TissueAdjoint(hidden, probes).trace(dur)
+    assert target.shape == (adj.n_steps(dur), 3)
+    loss, g, dgx, dgy = adj.loss_and_gradient(dur, target)
+    # The loss is that of the forward trace
+    tr = adj.trace(dur)
+    assert loss == pytest.approx(np.sum((tr - target) ** 2), rel=1e-12)
+
+    def loss_gx(gx):
+        s.set_conductance(gx)
+        r = adj.loss_and_gradient(dur, target)[0]
+        s.set_conductance(9.5)
+        return r
+    d = fd(loss_gx, 9.5, 9.5e-6)
+    print('dL/dgx adjoint %.10g fd %.10g' % (dgx, d), flush=True)
+    assert rel(dgx, d) < 1e-5
+
+    base = {k: np.array(v) for k, v in
+            ((var.qname(), vals) for var, vals in s._fields.items())}
+    for name in ('ina.gNa', 'ica.gCa'):
+        for c in (2, 11, 26):
+            def loss_f(v):
+                x = base[name].copy()
+                x[c] = v
+                s.set_field(name, x)
+                r = adj.loss_and_gradient(dur, target)[0]
+                s.set_field(name, base[name])
+                return r
+            v0 = base[name][c]
+            # Relative step 1e-4: truncation error ~1e-8 relative, while the
+            # rounding noise of the loss (~1e-16 L / h) stays far below the
+            # smallest gradients checked
+            h = v0 * 1e-4
+            d = fd(loss_f, v0, h)
+            print('dL/d%s[%d] adjoint %.10g fd %.10g (L %.6g)'
+                  % (name, c, g[name][c], d, loss), flush=True)
+            assert rel(g[name][c], d) < 1e-5 or \
+                abs(g[name][c] - d) < 1e-12 * loss / h
+
+
+def test_adjoint_sheet_rl_fd_fp64():
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=(12, 10), precision=64, rl=True)
+    s.set_step_size(0.01)
+    s.set_conductance(9.0, 7.0)
+    s.set_paced_cells(2, 10, 0, 0)
+    rng = np.random.default_rng(4)
+    s.set_field('ina.gNa', 16 * rng.uniform(0.8, 1.2, (10, 12)))
+    probes = [(5, 5), (10, 2), (11, 9)]
+    adj = TissueAdjoint(s, probes, checkpoint_every=50)
+    dur = 6.0
+    target = np.full((adj.n_steps(dur), 3), -20.0)
+    loss, g, dgx, dgy = adj.loss_and_gradient(dur, target)
+
+    def lg(gx, gy):
+        s.set_conductance(gx, gy)
+        r = adj.loss_and_gradient(dur, target)[0]
+        s.set_conductance(9.0, 7.0)
+        return r
+    dx = fd(lambda v: lg(v, 7.0), 9.0, 9e-6)
+    dy = fd(lambda v: lg(9.0, v), 7.0, 7e-6)
+    print('sheet dgx %.10g fd %.10g; dgy %.10g fd %.10g' % (dgx, dx, dgy, dy),
+          flush=True)
+    assert rel(dgx, dx) < 1e-5 and rel(dgy, dy) < 1e-5
+    base = np.array(s._fields[s._model.get('ina.gNa')]).reshape(10, 12)
+    for (x, y) in [(5, 5), (3, 4)]:
+        def lf(v):
+            b = base.copy()
+            b[y, x] = v
+            s.set_field('ina.gNa', b)
+            r = adj.loss_and_gradient(dur, target)[0]
+            s.set_field('ina.gNa', base)
+            return r
+        d = fd(lf, base[y, x], base[y, x] * 1e-4)
+        print('sheet dgNa[%d,%d] %.10g fd %.10g' % (y, x, g['ina.gNa'][y, x],
+                                                   d), flush=True)
+        assert rel(g['ina.gNa'][y, x], d) < 1e-5
+
+
+def test_adjoint_graph_scale_fd_fp64():
+    rng = np.random.default_rng(9)
+    n = 40
+    conns = [(i, i + 1, float(rng.uniform(5, 12))) for i in range(n - 1)]
+    conns += [(0, 20, 6.0), (5, 33, 8.0)]
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=n, precision=64)
+    s.set_connections(conns)
+    s.set_paced_cell_list([0, 1])
+    adj = TissueAdjoint(s, [10, 30], checkpoint_every=100)
+    dur = 5.0
+    target = np.full((adj.n_steps(dur), 2), -50.0)
+    loss, g, ds, _ = adj.loss_and_gradient(dur, target)
+
+    def ls(f):
+        s.set_connections([(i, j, c * f) for i, j, c in conns])
+        r = adj.loss_and_gradient(dur, target)[0]
+        s.set_connections(conns)
+        return r
+    d = fd(ls, 1.0, 1e-6)
+    print('graph dL/dscale %.10g fd %.10g' % (ds, d), flush=True)
+    assert rel(ds, d) < 1e-5
+
+
+def test_adjoint_linear_model_closed_form():
+    # dot(x) = -k x + u with per-cell fields k and u, forward Euler:
+    # x_{n+1} = (1 - k dt) x_n + dt u; dx/dk and dx/du by recursion
+    src = '''[[model]]
+c.x = 1.5
+[c]
+dot(x) = -k * x + u
+k = 0.5
+u = 0.2
+t = 0 bind time
+'''
+    m = myokit.parse_model(src)
+    n = 4
+    s = myokit.SimulationOpenCL(m, None, ncells=n, diffusion=False,
+                                precision=64)
+    dt = 0.01
+    s.set_step_size(dt)
+    k = np.array([0.5, 1.0, 2.0, 0.1])
+    u = np.array([0.2, -0.3, 1.0, 0.0])
+    s.set_field('c.k', k)
+    s.set_field('c.u', u)
+    adj = TissueAdjoint(s, [0, 1, 2, 3], checkpoint_every=7)
+    N = adj.n_steps(1.0)
+    assert N == 100
+    rng = np.random.default_rng(0)
+    target = rng.uniform(0, 1, (N, n))
+    loss, g, _, _ = adj.loss_and_gradient(1.0, target)
+    x = np.full(n, 1.5)
+    dk = np.zeros(n)
+    du = np.zeros(n)
+    L = 0.0
+    gk = np.zeros(n)
+    gu = np.zeros(n)
+    for i in range(N):
+        dk, du = (1 - k * dt) * dk - dt * x, (1 - k * dt) * du + dt
+        x = (1 - k * dt) * x + dt * u
+        r = x - target[i]
+        L += np.sum(r * r)
+        gk += 2 * r * dk
+        gu += 2 * r * du
+    assert loss == pytest.approx(L, rel=1e-12)
+    np.testing.assert_allclose(g['c.k'], gk, rtol=1e-10)
+    np.testing.assert_allclose(g['c.u'], gu, rtol=1e-10)
+
+
+def test_adjoint_single_precision_against_double():
+    dur = 3.0
+    probes = [5, 20]
+    out = {}
+    for prec in (32, 64):
+        s = cable(n=30, precision=prec)
+        adj = TissueAdjoint(s, probes, checkpoint_every=100)
+        target = np.full((adj.n_steps(dur), 2), -30.0)
+        out[prec] = adj.loss_and_gradient(dur, target)
+    l32, g32, x32, _ = out[32]
+    l64, g64, x64, _ = out[64]
+    print('fp32 loss %.8g fp64 %.8g; dgx %.8g / %.8g' % (l32, l64, x32, x64),
+          flush=True)
+    assert rel(l32, l64) < 1e-4
+    assert rel(x32, x64) < 1e-2
+    for name in g64:
+        err = np.max(np.abs(g32[name] - g64[name])) / np.max(np.abs(g64[name]))
+        assert err < 1e-2, name
+
+
+def test_torch_autograd_bridge():
+    torch = pytest.importorskip('torch')
+    from myokit_amd.adjoint import TissueLoss
+    s = cable(n=20)
+    adj = TissueAdjoint(s, [4, 15], checkpoint_every=50)
+    dur = 2.0
+    target = torch.full((adj.n_steps(dur), 2), -40.0, dtype=torch.float64)
+    gna = torch.tensor(16 * np.linspace(0.9, 1.1, 20), requires_grad=True)
+    gca = torch.tensor(0.09 * np.ones(20), requires_grad=True)
+    gx = torch.tensor(9.5, dtype=torch.float64, requires_grad=True)
+    gy = torch.tensor(5.0, dtype=torch.float64, requires_grad=True)
+    loss = TissueLoss.apply(adj, dur, target, gx, gy, gna, gca)
+    loss.backward()
+    l2, g, dgx, _ = adj.loss_and_gradient(dur, target.numpy())
+    assert float(loss) == pytest.approx(l2, rel=1e-14)
+    np.testing.assert_allclose(gna.grad.numpy(), g['ina.gNa'], rtol=1e-12)
+    np.testing.assert_allclose(gca.grad.numpy(), g['ica.gCa'], rtol=1e-12)
+    assert float(gx.grad) == pytest.approx(dgx, rel=1e-12)
