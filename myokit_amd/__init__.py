@@ -9,10 +9,13 @@ precision and log flags, the exception types, and ``SimulationOpenCL``
 itself, whose kernels are generated per model and compiled with hiprtc.
 """
 from ._err import (  # noqa
-    CyclicalDependencyError, DeviceError, FindNanError, IllegalReferenceError,
+    CyclicalDependencyError, DataBlockReadError, DataLogReadError,
+    DeviceError, FindNanError,
+    IllegalReferenceError,
     IntegrityError, MyokitError, NumericalError, ParseError,
     ProtocolEventError, SimulationCancelledError, SimulationError,
     SimultaneousProtocolEventError)
+from ._datablock import DataBlock1d, DataBlock2d  # noqa
 from ._datalog import (  # noqa
     DOUBLE_PRECISION, LOG_ALL, LOG_BOUND, LOG_DERIV, LOG_INTER, LOG_NONE,
     LOG_STATE, SINGLE_PRECISION, DataLog, prepare_log, split_key)

@@ -12,6 +12,7 @@ appends at large cell counts.
 ``prepare_log`` follows ``myokit.prepare_log`` (``_datalog.py:1783-2193``):
 flags, lists of names, and existing logs, with the same checks.
 """
+import os as _os
 import re
 from collections import OrderedDict
 
@@ -30,6 +31,15 @@ SINGLE_PRECISION = 32
 DOUBLE_PRECISION = 64
 
 _ID_NAME = re.compile(r'(\d+\.)+')
+
+_README = """Log file written by DataLog.save
+
+structure.txt: number of fields, length of each field, type code ('d' for
+64-bit or 'f' for 32-bit floats), the name of the time field (or an empty
+line), then one field name per line.
+data.bin: the fields in the order of structure.txt, each as a little-endian
+array of the given type and length.
+"""
 
 
 def split_key(key):
@@ -87,11 +97,10 @@ class DataLog(OrderedDict):
         return OrderedDict.__len__(self)
 
     def has_nan(self):
-        """True if any column contains a NaN or infinity as its last value
-        or anywhere."""
+        """True if any column has a NaN as its final logged value
+        (``DataLog.has_nan``, ``myokit/_datalog.py:366-374``)."""
         for v in self.values():
-            a = np.asarray(v, dtype=float)
-            if a.size and not np.all(np.isfinite(a)):
+            if len(v) > 0 and np.isnan(v[-1]):
                 return True
         return False
 
@@ -134,6 +143,68 @@ class DataLog(OrderedDict):
         else:
             for x in values:
                 cur.append(x)
+
+    def save(self, filename, precision=DOUBLE_PRECISION):
+        """Writes the log to a zip file in myokit's binary DataLog format
+        (``DataLog.save``, ``myokit/_datalog.py:1051-1126``):
+        ``structure.txt`` holds the number of columns, their length, the
+        type code (``d`` or ``f``), the time key and the column names, one
+        per line; ``data.bin`` the columns one after another as
+        little-endian arrays of that type."""
+        import zipfile
+        self.validate()
+        code = 'd' if precision == DOUBLE_PRECISION else 'f'
+        dt = np.dtype('<f8' if code == 'd' else '<f4')
+        head = [str(len(self)), str(self.length()), code, self._time or '']
+        body = []
+        for k, v in self.items():
+            head.append(k)
+            body.append(np.ascontiguousarray(v, dtype=dt).tobytes())
+        with zipfile.ZipFile(_os.path.expanduser(filename), 'w',
+                             zipfile.ZIP_DEFLATED) as f:
+            f.writestr('data.bin', b''.join(body))
+            f.writestr('structure.txt', '\n'.join(head).encode('utf8'))
+            f.writestr('readme.txt', _README.encode('utf8'))
+
+    @staticmethod
+    def load(filename, progress=None, msg='Loading DataLog'):
+        """Reads a log written by :meth:`save` (or by myokit's
+        ``DataLog.save``, ``myokit/_datalog.py:525-668``). Columns are numpy
+        arrays of the stored type."""
+        import zipfile
+        from ._err import DataLogReadError
+        try:
+            with zipfile.ZipFile(_os.path.expanduser(filename), 'r') as f:
+                try:
+                    head = f.read('structure.txt').decode('utf8')
+                    body = f.read('data.bin')
+                except KeyError:
+                    raise DataLogReadError('Invalid log file format.')
+        except zipfile.BadZipfile:
+            raise DataLogReadError('Unable to read log: bad zip file.')
+        lines = head.splitlines()
+        if len(lines) < 4:
+            raise DataLogReadError('Invalid log file format.')
+        n, size, code, time = int(lines[0]), int(lines[1]), lines[2], lines[3]
+        fields = lines[4:]
+        if len(fields) != n:
+            raise DataLogReadError('Invalid number of fields specified.')
+        if size < 0:
+            raise DataLogReadError('Invalid data size: ' + str(size) + '.')
+        if code not in ('d', 'f'):
+            raise DataLogReadError('Invalid data type: "' + code + '".')
+        dt = np.dtype('<f8' if code == 'd' else '<f4')
+        if n * size * dt.itemsize > len(body):
+            raise DataLogReadError(
+                'Header indicates larger data size than found in body.')
+        log = DataLog()
+        for k, name in enumerate(fields):
+            a = np.frombuffer(body, dtype=dt, count=size,
+                              offset=k * size * dt.itemsize)
+            log[name] = a.astype(dt.newbyteorder('='))
+        if time:
+            log._time = time
+        return log
 
     def trim(self, n):
         """Keeps the first ``n`` entries of every column."""

@@ -68,3 +68,11 @@ class FindNanError(MyokitError):
 
 class DeviceError(MyokitError, RuntimeError):
     """Raised when the HIP device library reports an error."""
+
+
+class DataLogReadError(MyokitError):
+    """Raised when a DataLog file cannot be read."""
+
+
+class DataBlockReadError(MyokitError):
+    """Raised when a DataBlock file cannot be read."""
