@@ -62,6 +62,32 @@ def cpu_baseline(nx, ny, sample_ms, threads):
     return r['steps'] * nx * ny / el, r['steps'], el
 
 
+def adjoint_leg(s, window_ms, checkpoint_every, reps):
+    """Times loss_and_gradient (forward + discrete adjoint, gradients for gx
+    and gy) over ``window_ms`` from the sheet's current state, 20 probe
+    cells on the diagonal, constant target."""
+    from myokit_amd.adjoint import TissueAdjoint
+    nx, ny = s._nx, s._ny
+    probes = [(int((k + 0.5) * nx / 20), int((k + 0.5) * ny / 20))
+              for k in range(20)]
+    n_steps = None
+    out = []
+    for K in checkpoint_every:
+        adj = TissueAdjoint(s, probes, checkpoint_every=K or 1 << 30)
+        if n_steps is None:
+            n_steps = adj.n_steps(window_ms)
+        target = np.full((n_steps, len(probes)), -80.0)
+        adj.loss_and_gradient(window_ms, target)      # build, allocate
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            loss, g, dgx, dgy = adj.loss_and_gradient(window_ms, target)
+        el = (time.perf_counter() - t0) / reps
+        assert np.isfinite(loss) and np.isfinite(dgx) and np.isfinite(dgy)
+        out.append((K or n_steps, el, adj.timing()))
+        adj.release()
+    return n_steps, out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -73,6 +99,9 @@ def main():
     ap.add_argument('--precision', type=int, default=32)
     ap.add_argument('--cpu-sample-ms', type=float, default=0.5)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--adjoint-ms', type=float, default=20.0,
+                    help='window of the fwd+adjoint timing (0: skip)')
+    ap.add_argument('--adjoint-reps', type=int, default=2)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -141,6 +170,21 @@ def main():
             'cells_per_launch': cells_per_launch,
         },
     }
+    if args.adjoint_ms > 0:
+        fwd_ms = el / args.steps * 1e3 * args.adjoint_ms / seg
+        n_adj, res = adjoint_leg(s, args.adjoint_ms, [0, 200],
+                                 args.adjoint_reps)
+        K0, t0, ph0 = res[0]
+        line['adjoint'] = {
+            'ratio': t0 * 1e3 / fwd_ms,
+            'window_ms': args.adjoint_ms, 'time_steps': n_adj,
+            'fwd_ms': fwd_ms, 'fwd_adjoint_ms': t0 * 1e3,
+            'checkpoint_every': K0, 'phases_ms': ph0,
+            'gradients': 'dL/dgx, dL/dgy; loss over 20 probe cells',
+            'with_checkpoints_every_200': {
+                'ratio': res[1][1] * 1e3 / fwd_ms,
+                'fwd_adjoint_ms': res[1][1] * 1e3, 'phases_ms': res[1][2]},
+        }
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         cv, csteps, cel = cpu_baseline(args.nx, args.ny, args.cpu_sample_ms,

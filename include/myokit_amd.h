@@ -161,7 +161,9 @@ int mkh_sim_clean(void* sim);
  *   "batch_steps"   max steps per mkh_sim_step call
  *   "log_rows"      device log buffer rows per batch
  *   "time_kernels"  1: bracket every launch with HIP events
- *   "block_threads" threads per workgroup for the next runs */
+ *   "block_threads" threads per workgroup for the next runs
+ *   "driver"        force a driver (MKH_DRIVER_*), -1 = automatic
+ *   "adj_release"   1: free the adjoint's trajectory and work buffers */
 int mkh_sim_set_option(void* sim, const char* key, long long value);
 
 /* Kernel timing since the last reset (time_kernels = 1): total milliseconds
@@ -178,15 +180,29 @@ int mkh_sim_reset_timing(void* sim);
  * over the N time steps of the run (one target row per step), and returns L,
  * dL/dfield as grad_fields[k * n_cells + c], and dL/dgx, dL/dgy (graphs:
  * dL/ds for all conductances scaled by s) in grad_g[0..1]. The kernel must
- * be generated with the adjoint; state is checkpointed every
- * checkpoint_every steps and recomputed segment by segment; loss and
- * gradients accumulate in double precision. */
+ * be generated with the adjoint. The forward pass keeps a checkpoint every
+ * checkpoint_every steps and all states of the last segment; earlier
+ * segments are recomputed from their checkpoints during the reverse pass
+ * (device memory: (checkpoint_every + 1) states plus one per checkpoint).
+ * Loss and gradients accumulate in double precision. */
 int mkh_adj_run(void* sim, double tmin, double tmax, double dt,
                 const double* events, long long n_events,
                 const long long* probes, long long n_probes,
                 const double* target, long long n_target_rows,
                 int checkpoint_every, double* loss, double* grad_fields,
                 double* grad_g);
+
+/* Page-locked host memory for log destinations (mkh_sim_set_log): rows
+ * copied into it leave the device while later steps compute. The reference
+ * appends each logged value to Python lists instead (openclsim.c:1120-1131);
+ * these two calls have no counterpart there. */
+int mkh_host_alloc(long long bytes, void** ptr);
+int mkh_host_free(void* ptr);
+
+/* Wall time in ms of the last mkh_adj_run's phases: ms3[0] forward with
+ * checkpoints, ms3[1] recomputation of earlier segments, ms3[2] reverse
+ * steps. No counterpart in the reference. */
+int mkh_adj_timing(void* sim, double* ms3);
 
 /* The forward quantity of mkh_adj_run's loss: V at the probe cells after
  * each of the N time steps of tmin..tmax, into trace[s * n_probes + p]

@@ -60,6 +60,21 @@ def dimco(*dims):
         yield tuple(int(i) for i in reversed(idx))
 
 
+def float_str(number, precision=DOUBLE_PRECISION):
+    """A number as text the way ``myokit.float.str`` writes it
+    (``myokit/float.py:64-103``): Python's repr when shorter than 10
+    characters, else full precision with a sign column
+    (``'{:< 1.17e}'``, single precision ``'{:< 1.9e}'``)."""
+    if isinstance(number, str):
+        return number
+    s = str(number)
+    if len(s) < 10:
+        return s
+    if precision == SINGLE_PRECISION:
+        return '{:< 1.9e}'.format(float(number))
+    return '{:< 1.17e}'.format(float(number))
+
+
 class DataLog(OrderedDict):
     """An ordered dict of logged columns, with an optional time key."""
 

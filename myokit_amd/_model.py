@@ -74,6 +74,28 @@ class _VarOwner:
             x = x._vars[p]
         return x
 
+    def move_variable(self, variable, new_parent, new_name=None):
+        """Moves a child variable (with its nested children) to another
+        component or variable (``Component.move_variable``,
+        ``myokit/_model_api.py``). References to it keep pointing at it: the
+        model's names are resolved to variables first."""
+        if variable._parent is not self:
+            raise ValueError('Variable <' + variable.qname()
+                             + '> is not a child of <' + self.qname() + '>.')
+        self.model().validate()
+        name = new_name or variable.name()
+        if not _NAME.match(name):
+            raise ValueError('Invalid variable name <' + str(name) + '>.')
+        if name in new_parent._vars:
+            raise IntegrityError(
+                'Duplicate variable name <' + name + '> in <'
+                + new_parent.qname() + '>.')
+        del self._vars[variable.name()]
+        variable._parent = new_parent
+        variable._name = name
+        new_parent._vars[name] = variable
+        self.model()._invalidate()
+
 
 class Variable(_VarOwner):
     """A model variable (state, constant, intermediary or bound input)."""

@@ -7,6 +7,8 @@ simulation is created, an error is raised.
 """
 import ctypes
 import os
+import threading
+import weakref
 
 from ._err import DeviceError
 
@@ -80,6 +82,9 @@ SIGNATURES = {
     'mkh_sim_driver': (_i, [_vp, _pi]),
     'mkh_adj_run': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll, _i,
                          _pd, _pd, _pd]),
+    'mkh_adj_timing': (_i, [_vp, _pd]),
+    'mkh_host_alloc': (_i, [_ll, ctypes.POINTER(ctypes.c_void_p)]),
+    'mkh_host_free': (_i, [_vp]),
     'mkh_adj_trace': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll]),
     'mkh_pacing_levels': (_i, [_pd, _ll, _d, _pd, _ll, _pd, _pd]),
     'mkh_plan': (_i, [_d, _d, _d, _pd, _ll, _d, _pd, _pd, _pd, _pi, _ll,
@@ -138,3 +143,60 @@ def device_info(device=0):
 def ptr(array, ctype):
     """Pointer to a contiguous numpy array's data."""
     return array.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class _PinnedPool:
+    """Page-locked host blocks (``mkh_host_alloc``) for log outputs, so that
+    log rows reach the host while later steps compute. An array from
+    :meth:`empty` gives its block back to the pool when the last view of it
+    is gone; the pool keeps at most ``keep`` bytes of free blocks."""
+
+    GRANULE = 1 << 20
+
+    def __init__(self, keep=2 << 30):
+        self.keep = keep
+        self._free = []         # (size, address)
+        self._kept = 0
+        self._lock = threading.Lock()
+
+    def empty(self, shape, dtype):
+        import numpy as np
+        dtype = np.dtype(dtype)
+        count = 1
+        for d in shape:
+            count *= int(d)
+        nbytes = count * dtype.itemsize
+        if nbytes == 0:
+            return np.empty(shape, dtype)
+        size = -(-nbytes // self.GRANULE) * self.GRANULE
+        addr = None
+        with self._lock:
+            # smallest free block that fits, if not more than 1.5x the need
+            best = None
+            for k, (sz, a) in enumerate(self._free):
+                if size <= sz <= size * 3 // 2 and (
+                        best is None or sz < self._free[best][0]):
+                    best = k
+            if best is not None:
+                size, addr = self._free.pop(best)
+                self._kept -= size
+        if addr is None:
+            p = ctypes.c_void_p()
+            check(lib().mkh_host_alloc(size, ctypes.byref(p)),
+                  'mkh_host_alloc')
+            addr = p.value
+        buf = (ctypes.c_char * size).from_address(addr)
+        weakref.finalize(buf, self._give_back, addr, size)
+        return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
+
+    def _give_back(self, addr, size):
+        with self._lock:
+            if self._kept + size <= self.keep:
+                self._free.append((size, addr))
+                self._kept += size
+                return
+        if _lib is not None:
+            _lib.mkh_host_free(addr)
+
+
+pinned = _PinnedPool()

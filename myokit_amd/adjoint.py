@@ -42,9 +42,6 @@ class TissueAdjoint:
         self.checkpoint_every = int(checkpoint_every)
         if self.checkpoint_every < 1:
             raise ValueError('checkpoint_every must be at least 1.')
-        if sim._connections is not None:
-            # Adjoint kernels use the global connection layout
-            sim.set_engine_option('driver', _native.DRIVER_GRID)
 
     def _events(self):
         sim = self.sim
@@ -104,6 +101,25 @@ class TissueAdjoint:
             grads[var.qname()] = gf[k * sim._ntotal:(k + 1) * sim._ntotal] \
                 .reshape(shape).copy()
         return loss.value, grads, gg[0], gg[1]
+
+    def timing(self):
+        """Wall time in ms of the last ``loss_and_gradient``'s phases:
+        ``{'forward', 'recompute', 'reverse'}``."""
+        ms = np.zeros(3)
+        eng = self.sim._engine
+        if eng is None:
+            return {'forward': 0.0, 'recompute': 0.0, 'reverse': 0.0}
+        _native.check(_native.lib().mkh_adj_timing(
+            eng.h, _native.ptr(ms, ctypes.c_double)), 'mkh_adj_timing')
+        return {'forward': ms[0], 'recompute': ms[1], 'reverse': ms[2]}
+
+    def release(self):
+        """Frees the device trajectory and work buffers kept between
+        calls."""
+        eng = self.sim._engine
+        if eng is not None:
+            _native.check(_native.lib().mkh_sim_set_option(
+                eng.h, b'adj_release', 1), 'set_option')
 
     def trace(self, duration):
         """V at the probes after every time step of a run of ``duration``

@@ -2,125 +2,287 @@
  * mk_adjoint.hip -- discrete adjoint kernels (appended to a generated kernel
  * source together with mk_cell_vjp when gradients are requested).
  *
- * For the step y_{s+1} = F(y_s) of the whole tissue, with the loss
- *   L = sum_s sum_p (V_s[probe p] - target[s-1][p])^2,   s = 1..N,
- * the adjoint recursion is lam_N = dL/dy_N and
- *   lam_s = (dF/dy_s)^T lam_{s+1} + dL/dy_s.
- * dF/dy_s has two parts: each cell's own update (mk_cell_vjp) and the
- * coupling through the diffusion current idiff_c = sum_j g_cj (V_c - V_j).
- * The transpose of the coupling applied to the per-cell adjoints abar of
- * idiff is the same weighted graph Laplacian, sum_j g_cj (abar_c - abar_j),
- * added to the V component. mk_adj_vjp computes the per-cell part;
- * mk_adj_couple adds the coupling, the loss gradient and the conductance
- * gradients. Gradient and loss accumulators are double precision in every
- * precision, and each cell accumulates its own entries (no atomics, so the
- * result does not depend on scheduling).
+ * For the step y_{s+1} = F_s(y_s) of the whole tissue, s = 0..N-1, and the
+ * loss
+ *   L = sum_{s=0..N-1} sum_p (V(y_{s+1})[probe p] - target[s][p])^2,
+ * the adjoint recursion is
+ *   lam_N = dL/dy_N,   lam_s = (dF_s/dy_s)^T lam_{s+1} + dL/dy_s,
+ * and a parameter gradient is sum_s (dF_s/dparam)^T lam_{s+1}.
+ * (dF_s/dy_s)^T lam_{s+1} has two parts: the cell's own update, whose
+ * transpose mk_cell_vjp applies (mu_s), and the coupling through the
+ * diffusion current idiff_c = sum_j g_cj (V_c - V_j). With abar_s the
+ * per-cell adjoint of idiff (also from mk_cell_vjp), the coupling part is
+ * the same weighted graph Laplacian applied to abar_s, added to the V
+ * component:  sum_j g_cj (abar_s[c] - abar_s[j]).
+ *
+ * One reverse step s assembles lam_{s+1} from mu_{s+1}, the Laplacian of
+ * abar_{s+1} and the loss term at y_{s+1}, adds the conductance-gradient
+ * terms of step s+1, then applies the cell VJP of step s at y_s, giving
+ * mu_s and abar_s. lam itself is never stored. Two drivers:
+ *
+ *   mk_adj_step   one reverse step per launch over all cells (any size); the
+ *                 Laplacian needs every cell's abar_{s+1}, hence the launch
+ *                 boundary between steps;
+ *   mk_adj_block  one workgroup per independent domain (cable, small sheet,
+ *                 graph component), every reverse step of a segment in one
+ *                 launch; V and abar of the domain live in LDS, one barrier
+ *                 per step.
+ *
+ * Field gradients accumulate per cell in double precision. Conductance
+ * gradients are summed over each workgroup (wave shuffles, then LDS) into
+ * one double per workgroup, so every partial sum has a single writer and a
+ * fixed order: results do not depend on scheduling.
  */
 typedef MkAdjT<Real> MkAdj;
 
-extern "C" __global__ void __launch_bounds__(MK_BT)
-mk_adj_vjp(const MkArgs a, const MkAdj j, const int s)
+/* Sum over the workgroup of a double per thread; valid in thread 0. */
+__device__ __forceinline__ double mk_block_sum(double v, double* lds)
 {
-    const long long c = (long long)blockIdx.x * MK_BT + threadIdx.x;
-    if (c >= a.n) return;
-    const MkStep p = a.steps[s];
-    Real y[MK_NS], lam[MK_NS], mu[MK_NS], fv[MK_NFA], fb[MK_NFA], ab;
 #pragma unroll
-    for (int i = 0; i < MK_NS; ++i) {
-        y[i] = j.yn[(long long)i * a.npad + c];
-        lam[i] = j.lam[(long long)i * a.npad + c];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < MK_BT / 64; ++k) r += lds[k];
     }
-#pragma unroll
-    for (int k = 0; k < MK_NF; ++k) fv[k] = a.fields[(long long)k * a.npad + c];
-    const long long ix = c % a.nx, iy = c / a.nx;
-    const Real* __restrict__ V = j.yn + (long long)MK_IVM * a.npad;
+    return r;
+}
+
+/* Coupling transpose at cell c for abar_{s+1} (ab) and V(y_{s+1}) (V1):
+ * returns sum_j g_cj (ab[c] - ab[j]) and adds the cell's terms of dL/dgx and
+ * dL/dgy (graphs: dL/dscale, half from each end of a connection) to cx,
+ * cy. Arrays are indexed as in the forward coupling: by cell on a grid or
+ * a single stencil domain; graphs by CSR row `row` with neighbour indices
+ * relative to the arrays, the cell itself at `self`. */
+__device__ __forceinline__ Real mk_adj_couple(
+    const MkArgs& a, const Real* __restrict__ V1, const Real* __restrict__ ab,
+    long long c, long long row, long long self, long long ix, long long iy,
+    double& cx, double& cy)
+{
+    Real lv = (Real)0;
+    (void)row; (void)self; (void)ix; (void)iy; (void)c;
+#if MK_COUPLING == 1
+    const Real ac = ab[c], v = V1[c];
+    Real ax = (Real)0, ay = (Real)0, xs = (Real)0, ys = (Real)0;
+    if (a.nx > 1) {
+        if (ix > 0) { ax += ac - ab[c - 1]; xs += v - V1[c - 1]; }
+        if (ix < a.nx - 1) { ax += ac - ab[c + 1]; xs += v - V1[c + 1]; }
+    }
+    if (a.ny > 1) {
+        if (iy > 0) { ay += ac - ab[c - a.nx]; ys += v - V1[c - a.nx]; }
+        if (iy < a.ny - 1) { ay += ac - ab[c + a.nx]; ys += v - V1[c + a.nx]; }
+    }
+    lv = a.gx * ax + a.gy * ay;
+    cx += (double)ac * (double)xs;
+    cy += (double)ac * (double)ys;
+#elif MK_COUPLING == 2
+    const Real ac = ab[c];
+    if (a.nx > 1) {
+        if (ix > 0) lv += a.gx_field[c - iy - 1] * (ac - ab[c - 1]);
+        if (ix < a.nx - 1) lv += a.gx_field[c - iy] * (ac - ab[c + 1]);
+    }
+    if (a.ny > 1) {
+        if (iy > 0) lv += a.gy_field[c - a.nx] * (ac - ab[c - a.nx]);
+        if (iy < a.ny - 1) lv += a.gy_field[c] * (ac - ab[c + a.nx]);
+    }
+#elif MK_COUPLING == 3
+    const Real ac = ab[self], v = V1[self];
+    for (int e = a.csr_ptr[row]; e < a.csr_ptr[row + 1]; ++e) {
+        int nb = a.csr_idx[e];
+        if (nb < 0) nb = ~nb;
+        const Real g = a.csr_g[e];
+        lv += g * (ac - ab[nb]);
+        cx += 0.5 * (double)g * ((double)ac - (double)ab[nb])
+              * ((double)v - (double)V1[nb]);
+    }
+#endif
+    return lv;
+}
+
+/* Loss term dL/dV at cell c for V(y_{s+1}) = v, target row `row`. */
+__device__ __forceinline__ Real mk_adj_loss(const MkAdj& j, long long c,
+                                            Real v, int row)
+{
+    if (row < 0) return (Real)0;
+    const int q = j.probe_of[c];
+    if (q < 0) return (Real)0;
+    const double d = (double)v - j.target[(long long)row * j.n_probe + q];
+    j.loss_acc[q] += d * d;
+    return (Real)(2.0 * d);
+}
+
+/* Cell VJP of step p at state y (own cell), V the potentials of y_s indexed
+ * as in the forward coupling. */
+__device__ __forceinline__ void mk_adj_cell(
+    const MkArgs& a, const MkStep& p, const Real* y, const Real* fv,
+    const Real* __restrict__ V, long long c, long long row, long long self,
+    long long ix, long long iy, const Real* lam, Real* mu, Real& abo,
+    Real* fb)
+{
+    (void)row; (void)self;
 #if MK_COUPLING == 1
     const Real idiff = mk_stencil(V, c, ix, iy, a.nx, a.ny, a.gx, a.gy);
 #elif MK_COUPLING == 2
-    const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy,
-                                 a.nx, a.ny);
+    const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy, a.nx,
+                                 a.ny);
 #elif MK_COUPLING == 3
-    const Real idiff = mk_graph(V, a.csr_ptr, a.csr_idx, a.csr_g, c, c);
+    const Real idiff = mk_graph(V, a.csr_ptr, a.csr_idx, a.csr_g, row, self);
 #else
     const Real idiff = (Real)0;
+    (void)V;
 #endif
     const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace : (Real)0;
-    mk_cell_vjp(y, p.t, p.dt, pace, idiff, fv, lam, mu, &ab, fb);
-#pragma unroll
-    for (int i = 0; i < MK_NS; ++i) j.mu[(long long)i * a.npad + c] = mu[i];
-    j.abar[c] = ab;
-#pragma unroll
-    for (int k = 0; k < MK_NF; ++k)
-        j.gfield[(long long)k * a.npad + c] += (double)fb[k];
+    mk_cell_vjp(y, p.t, p.dt, pace, idiff, fv, lam, mu, &abo, fb);
 }
 
+/* ---------------------------------------------------------------------- */
 extern "C" __global__ void __launch_bounds__(MK_BT)
-mk_adj_couple(const MkArgs a, const MkAdj j, const int s)
+mk_adj_step(const MkArgs a, const MkAdj j, const int s)
 {
+    __shared__ double red[2][MK_BT / 64];
     const long long c = (long long)blockIdx.x * MK_BT + threadIdx.x;
-    if (c >= a.n) return;
-    (void)s;
+    double cx = 0.0, cy = 0.0;
+    if (c < a.n) {
+        const long long ix = c % a.nx, iy = c / a.nx;
+        Real lam[MK_NS];
 #pragma unroll
-    for (int i = 0; i < MK_NS; ++i)
-        j.lam_out[(long long)i * a.npad + c] =
-            j.mu ? j.mu[(long long)i * a.npad + c] : (Real)0;
-    const Real* __restrict__ V = j.yn + (long long)MK_IVM * a.npad;
-    const Real v = V[c];
-    Real lv = (Real)0;
-    if (j.mu) {
-        const Real ac = j.abar[c];
-#if MK_COUPLING == 1
-        const long long ix = c % a.nx, iy = c / a.nx;
-        Real ax = (Real)0, xs = (Real)0, ay = (Real)0, ys = (Real)0;
-        if (a.nx > 1) {
-            if (ix > 0) { ax += ac - j.abar[c - 1]; xs += v - V[c - 1]; }
-            if (ix < a.nx - 1) { ax += ac - j.abar[c + 1]; xs += v - V[c + 1]; }
+        for (int i = 0; i < MK_NS; ++i)
+            lam[i] = j.mu_in ? j.mu_in[(long long)i * a.npad + c] : (Real)0;
+        const Real* __restrict__ V1 = j.yn1 + (long long)MK_IVM * a.npad;
+        Real lv = mk_adj_loss(j, c, V1[c], j.target_row);
+        if (j.ab_in)
+            lv += mk_adj_couple(a, V1, j.ab_in, c, c, c, ix, iy, cx, cy);
+        lam[MK_IVM] += lv;
+        if (j.yn) {
+            Real y[MK_NS], mu[MK_NS], fv[MK_NFA], fb[MK_NFA], abo;
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                y[i] = j.yn[(long long)i * a.npad + c];
+#pragma unroll
+            for (int k = 0; k < MK_NF; ++k)
+                fv[k] = a.fields[(long long)k * a.npad + c];
+            mk_adj_cell(a, a.steps[s], y, fv,
+                        j.yn + (long long)MK_IVM * a.npad, c, c, c, ix, iy,
+                        lam, mu, abo, fb);
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                j.mu_out[(long long)i * a.npad + c] = mu[i];
+            j.ab_out[c] = abo;
+#pragma unroll
+            for (int k = 0; k < MK_NF; ++k)
+                j.gfield[(long long)k * a.npad + c] += (double)fb[k];
         }
-        if (a.ny > 1) {
-            if (iy > 0) { ay += ac - j.abar[c - a.nx]; ys += v - V[c - a.nx]; }
-            if (iy < a.ny - 1) {
-                ay += ac - j.abar[c + a.nx];
-                ys += v - V[c + a.nx];
-            }
+    }
+#if MK_COUPLING == 1 || MK_COUPLING == 3
+    if (j.ab_in) {
+        const double sx = mk_block_sum(cx, red[0]);
+        const double sy = mk_block_sum(cy, red[1]);
+        if (threadIdx.x == 0) {
+            j.gx_acc[blockIdx.x] += sx;
+            j.gy_acc[blockIdx.x] += sy;
         }
-        lv = a.gx * ax + a.gy * ay;
-        j.gx_acc[c] += (double)ac * (double)xs;
-        j.gy_acc[c] += (double)ac * (double)ys;
-#elif MK_COUPLING == 2
-        const long long ix = c % a.nx, iy = c / a.nx;
-        if (a.nx > 1) {
-            if (ix > 0) lv += a.gx_field[c - iy - 1] * (ac - j.abar[c - 1]);
-            if (ix < a.nx - 1) lv += a.gx_field[c - iy] * (ac - j.abar[c + 1]);
-        }
-        if (a.ny > 1) {
-            if (iy > 0) lv += a.gy_field[c - a.nx] * (ac - j.abar[c - a.nx]);
-            if (iy < a.ny - 1) lv += a.gy_field[c] * (ac - j.abar[c + a.nx]);
-        }
-#elif MK_COUPLING == 3
-        double gs = 0;
-        for (int e = a.csr_ptr[c]; e < a.csr_ptr[c + 1]; ++e) {
-            int nb = a.csr_idx[e];
-            if (nb < 0) nb = ~nb;
-            const Real g = a.csr_g[e];
-            lv += g * (ac - j.abar[nb]);
-            gs += (double)g * ((double)ac - (double)j.abar[nb])
-                  * ((double)v - (double)V[nb]);
-        }
-        /* each connection is seen from both ends: half each */
-        j.gx_acc[c] += 0.5 * gs;
+    }
+#else
+    (void)red;
 #endif
-    }
-    if (j.target_row >= 0 && j.probe_of[c] >= 0) {
-        const int q = j.probe_of[c];
-        const double d = (double)v - j.target[(long long)j.target_row
-                                              * j.n_probe + q];
-        lv += (Real)(2.0 * d);
-        j.loss_acc[q] += d * d;
-    }
-    j.lam_out[(long long)MK_IVM * a.npad + c] += lv;
 }
 
-/* V at the probe cells after step s (trace row s). */
+/* Every reverse step of one segment [s0, s1) for one domain per workgroup.
+ * j.yn is the segment's trajectory: y_{s0 + k} at j.yn + k * n_state * npad
+ * for k = 0..s1-s0. j.mu_out holds mu of each cell (read and written in
+ * place: a cell only reads its own); j.ab_out carries abar of each cell
+ * from one launch to the next. flags: bit 0 = this segment ends the run (no
+ * mu or abar of step N yet), bit 1 = this segment starts the run (add the
+ * conductance terms of step 0 at the end). */
+extern "C" __global__ void __launch_bounds__(MK_BT)
+mk_adj_block(const MkArgs a, const MkAdj j, const int s0, const int s1,
+             const int flags)
+{
+    __shared__ Real vsh[2][MK_BCELLS];
+    __shared__ Real ash[2][MK_BCELLS];
+    __shared__ double red[2][MK_BT / 64];
+    const long long off = a.dom_off[blockIdx.x];
+    const int len = (int)(a.dom_off[blockIdx.x + 1] - off);
+    const long long SN = (long long)MK_NS * a.npad;
+    const bool last_seg = flags & 1, first_seg = flags & 2;
+    double cx = 0.0, cy = 0.0;
+    /* V(y_{s1}) and abar_{s1} (zero at the end of the run) */
+    int cur = 0;
+    for (int li = threadIdx.x; li < len; li += MK_BT) {
+        const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+        vsh[0][li] = j.yn[(long long)(s1 - s0) * SN
+                          + (long long)MK_IVM * a.npad + c];
+        ash[0][li] = last_seg ? (Real)0 : j.ab_out[c];
+    }
+    __syncthreads();
+    for (int s = s1 - 1; s >= s0; --s) {
+        const Real* __restrict__ ys = j.yn + (long long)(s - s0) * SN;
+        /* V(y_s) of the domain into the other buffer */
+        for (int li = threadIdx.x; li < len; li += MK_BT) {
+            const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+            vsh[cur ^ 1][li] = ys[(long long)MK_IVM * a.npad + c];
+        }
+        __syncthreads();
+        const MkStep p = a.steps[s];
+        const bool fresh = last_seg && s == s1 - 1;
+        for (int li = threadIdx.x; li < len; li += MK_BT) {
+            const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+            const long long ix = c % a.nx, iy = c / a.nx;
+            Real lam[MK_NS], y[MK_NS], mu[MK_NS], fv[MK_NFA], fb[MK_NFA], abo;
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                lam[i] = fresh ? (Real)0 : j.mu_out[(long long)i * a.npad + c];
+            Real lv = mk_adj_loss(j, c, vsh[cur][li], s);
+            if (!fresh)
+                lv += mk_adj_couple(a, vsh[cur], ash[cur], li, off + li, li,
+                                    ix, iy, cx, cy);
+            lam[MK_IVM] += lv;
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                y[i] = ys[(long long)i * a.npad + c];
+#pragma unroll
+            for (int k = 0; k < MK_NF; ++k)
+                fv[k] = a.fields[(long long)k * a.npad + c];
+            mk_adj_cell(a, p, y, fv, vsh[cur ^ 1], li, off + li, li, ix, iy,
+                        lam, mu, abo, fb);
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                j.mu_out[(long long)i * a.npad + c] = mu[i];
+            ash[cur ^ 1][li] = abo;
+#pragma unroll
+            for (int k = 0; k < MK_NF; ++k)
+                j.gfield[(long long)k * a.npad + c] += (double)fb[k];
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+    /* Carry abar_{s0} to the previous segment; at the start of the run add
+     * the conductance terms of step 0 instead. */
+    for (int li = threadIdx.x; li < len; li += MK_BT) {
+        const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+        if (first_seg) {
+            const long long ix = c % a.nx, iy = c / a.nx;
+            (void)mk_adj_couple(a, vsh[cur], ash[cur], li, off + li, li, ix,
+                                iy, cx, cy);
+        } else {
+            j.ab_out[c] = ash[cur][li];
+        }
+    }
+#if MK_COUPLING == 1 || MK_COUPLING == 3
+    const double sx = mk_block_sum(cx, red[0]);
+    const double sy = mk_block_sum(cy, red[1]);
+    if (threadIdx.x == 0) {
+        j.gx_acc[blockIdx.x] += sx;
+        j.gy_acc[blockIdx.x] += sy;
+    }
+#else
+    (void)red; (void)cx; (void)cy;
+#endif
+}
+
+/* V at the probe cells of state yn (trace row s). */
 extern "C" __global__ void __launch_bounds__(64)
 mk_adj_probe(const MkArgs a, const MkAdj j, const int s)
 {

@@ -50,21 +50,23 @@ struct MkArgsT {
 /* Discrete adjoint of one step (see mk_adjoint.hip). */
 template <typename R>
 struct MkAdjT {
-    const R* yn;            /* state at t_s (SoA) */
-    const R* lam;           /* adjoint of the state at t_{s+1} (SoA) */
-    R* mu;                  /* (dF/dy)^T lam per cell (SoA); NULL = zero */
-    R* abar;                /* adjoint of each cell's diffusion current */
-    R* lam_out;             /* adjoint of the state at t_s (SoA) */
+    const R* yn;            /* y_s, state before step s (SoA); NULL in the
+                               closing pass */
+    const R* yn1;           /* y_{s+1} = F(y_s) (SoA) */
+    const R* mu_in;         /* (dF_{s+1}/dy)^T lam_{s+2} per cell; NULL = 0 */
+    const R* ab_in;         /* adjoint of idiff at step s+1; NULL = 0 */
+    R* mu_out;              /* (dF_s/dy)^T lam_{s+1} per cell (SoA) */
+    R* ab_out;              /* adjoint of idiff at step s */
     double* gfield;         /* [n_field][npad] field gradients (accumulated) */
-    double* gx_acc;         /* [npad] d/dgx contributions (accumulated) */
-    double* gy_acc;         /* [npad] d/dgy contributions (accumulated) */
+    double* gx_acc;         /* [workgroups] d/dgx partial sums (accumulated) */
+    double* gy_acc;         /* [workgroups] d/dgy partial sums (accumulated) */
     double* loss_acc;       /* [n_probe] squared errors (accumulated) */
     const int* probe_of;    /* [npad] probe index of each cell, or -1 */
     const double* target;   /* [rows][n_probe] target potentials */
     const long long* probes;/* [n_probe] probe cells */
     double* trace;          /* [rows][n_probe] V at the probes, or NULL */
     int n_probe;
-    int target_row;         /* target row for the loss at t_s, or -1 */
+    int target_row;         /* target row compared with V(y_{s+1}), or -1 */
 };
 
 /* Persistent sheet driver: neighbour exchange between workgroup tiles. */
@@ -77,6 +79,8 @@ struct MkSheetT {
     int tiles_y;      /* tiles along y (MK_STH cells each) */
     int g0;           /* steps completed before this launch */
     int spin_limit;   /* polls before a wait is abandoned */
+    R* traj;          /* if set: the state after local step i is stored at
+                         traj + i * n_state * npad (SoA), for the adjoint */
 };
 
 #endif

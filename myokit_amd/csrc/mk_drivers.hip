@@ -17,7 +17,7 @@
  * with the state at t, the intermediaries and idiff written to the log at
  * log steps. The reference runs this as 2-3 kernels per step with the state
  * in AoS layout; here the coupling and the update are fused in one pass over
- * SoA state, and three drivers cover the geometries:
+ * SoA state, and four drivers cover the geometries:
  *
  *   mk_cells  uncoupled cells: one thread per cell keeps its state in
  *             registers across every step of the launch (no synchronisation);
@@ -26,7 +26,11 @@
  *             with one barrier per step, all steps of the launch;
  *   mk_grid   any size: one fused step per launch, V of neighbours read from
  *             the previous state buffer, new state written to the other
- *             buffer (ping-pong).
+ *             buffer (ping-pong);
+ *   mk_sheet  large homogeneous 2D sheets: a persistent cooperative launch,
+ *             one 64 x 16 tile per workgroup with its state in registers
+ *             for every step of the launch; tile edges are exchanged
+ *             through L2 with a step counter per tile.
  *
  * Halting: at a log step, the thread of cell 0 records the log row in
  * halt[0] if state 0 is NaN (openclsim.c:1087-1089). The host ends launches
@@ -120,6 +124,17 @@ __device__ __forceinline__ void mk_write_log(
     }
 }
 
+/* Adjoint support: the state after local step i of a launch goes to
+ * traj + i * n_state * npad (SoA), the trajectory the reverse pass reads. */
+__device__ __forceinline__ void mk_store_traj(const MkArgs& a, Real* traj,
+                                              int i, long long c,
+                                              const Real* y)
+{
+    Real* __restrict__ tr = traj + (long long)i * MK_NS * a.npad + c;
+#pragma unroll
+    for (int st = 0; st < MK_NS; ++st) tr[(long long)st * a.npad] = y[st];
+}
+
 __device__ __forceinline__ bool mk_halted(const MkArgs& a, int own_row)
 {
     const int h = __atomic_load_n(a.halt, __ATOMIC_RELAXED);
@@ -130,7 +145,7 @@ __device__ __forceinline__ bool mk_halted(const MkArgs& a, int own_row)
  * Uncoupled cells: one thread per cell, state in registers for all steps.
  * ---------------------------------------------------------------------- */
 extern "C" __global__ void __launch_bounds__(MK_BT)
-mk_cells(const MkArgs a, const int s0, const int s1)
+mk_cells(const MkArgs a, const int s0, const int s1, Real* const traj)
 {
     if (mk_halted(a, a.steps[s1 - 1].row)) return;
     const long long c = (long long)blockIdx.x * MK_BT + threadIdx.x;
@@ -153,6 +168,7 @@ mk_cells(const MkArgs a, const int s0, const int s1)
          * copies, written out only at log steps */
         mk_cell<true>(y, p.t, p.dt, pace, (Real)0, fv, q);
         if (p.row >= 0) mk_write_log(a, c, p.row, q);
+        if (traj) mk_store_traj(a, traj, s - s0, c, y);
     }
 #pragma unroll
     for (int i = 0; i < MK_NS; ++i) a.state_a[(long long)i * a.npad + c] = y[i];
@@ -162,7 +178,7 @@ mk_cells(const MkArgs a, const int s0, const int s1)
  * One workgroup per domain; V exchanged through LDS, one barrier per step.
  * ---------------------------------------------------------------------- */
 extern "C" __global__ void __launch_bounds__(MK_BT)
-mk_block(const MkArgs a, const int s0, const int s1)
+mk_block(const MkArgs a, const int s0, const int s1, Real* const traj)
 {
     __shared__ Real vsh[2][MK_BCELLS];
     const bool stop = mk_halted(a, a.steps[s1 - 1].row);
@@ -215,6 +231,7 @@ mk_block(const MkArgs a, const int s0, const int s1)
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i)
                 a.state_a[(long long)i * a.npad + c] = y[i];
+            if (traj) mk_store_traj(a, traj, s - s0, c, y);
             vsh[cur ^ 1][li] = y[MK_IVM];
         }
         cur ^= 1;
@@ -486,6 +503,7 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
                  * copies, written out only at log steps */
                 mk_cell<true>(y[k], p.t, p.dt, pace, id, fv[k], q);
                 if (p.row >= 0) mk_write_log(a, c, p.row, q);
+                if (sh.traj) mk_store_traj(a, sh.traj, i, c, y[k]);
                 const Real vn = y[k][MK_IVM];
                 vt[cur ^ 1][r + 1][lx + 1] = vn;
                 if (!last) {

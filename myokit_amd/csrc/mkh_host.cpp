@@ -18,6 +18,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -60,7 +61,7 @@ struct Program {
     hipModule_t module = nullptr;
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
     hipFunction_t f_sheet = nullptr;
-    hipFunction_t f_adj_vjp = nullptr, f_adj_couple = nullptr,
+    hipFunction_t f_adj_step = nullptr, f_adj_block = nullptr,
                   f_adj_probe = nullptr;
     std::vector<char> code;
 };
@@ -169,6 +170,9 @@ template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release()
     {
@@ -215,6 +219,7 @@ struct SimBase {
     virtual int adj_run(double, double, double, const double*, long long,
                         const long long*, long long, const double*, long long,
                         int, double*, double*, double*, double*) = 0;
+    virtual int adj_timing(double*) = 0;
 };
 
 template <typename R>
@@ -272,10 +277,19 @@ struct Sim : SimBase {
     long long launches = 0, cell_steps = 0;
     std::vector<hipEvent_t> ev_pool;
 
+    /* Log rows go to the host on a second stream (default priority), each
+     * as soon as the launch that wrote it has finished, while the next
+     * launches run. row_ev is a small ring of "row ready" events. */
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t row_ev[16] = {};
+
     ~Sim() override
     {
         if (h_steps) (void)hipHostFree(h_steps);
         for (auto e : ev_pool) (void)hipEventDestroy(e);
+        for (auto e : row_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -288,6 +302,9 @@ struct Sim : SimBase {
         max_block_cells = cfg.precision == 64 ? 2048 : 4096;
         MKH_HIP(hipSetDevice(cfg.device));
         MKH_HIP(hipStreamCreate(&stream));
+        MKH_HIP(hipStreamCreate(&copy_stream));
+        for (auto& e : row_ev)
+            MKH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         MKH_HIP(state[0].alloc((size_t)cfg.n_state * npad));
         MKH_HIP(state[1].alloc((size_t)cfg.n_state * npad));
         MKH_HIP(hipMemset(state[0].p, 0, state[0].n * sizeof(R)));
@@ -718,6 +735,20 @@ struct Sim : SimBase {
         return a;
     }
 
+    MkSheetT<R> sheet_args()
+    {
+        MkSheetT<R> sh;
+        std::memset(&sh, 0, sizeof(sh));
+        sh.edges = sh_edges.p;
+        sh.cnt = sh_cnt.p;
+        sh.err = sh_err.p;
+        sh.tiles_x = sh_tx;
+        sh.tiles_y = sh_ty;
+        sh.spin_limit = 1 << 22;
+        sh.traj = nullptr;
+        return sh;
+    }
+
     hipEvent_t event(size_t k)
     {
         while (ev_pool.size() <= k) {
@@ -770,6 +801,21 @@ struct Sim : SimBase {
         }
         MKH_HIP(hipMemcpyAsync(steps.p, h_steps, ns * sizeof(Step),
                                hipMemcpyHostToDevice, stream));
+        /* After the launch that ends with step s: if s logs a row, copy the
+         * row to the caller's buffer on the copy stream. */
+        const bool copy_rows = nr > 0 && ncols > 0 && dst_values;
+        auto emit_row = [&](long long s) -> int {
+            const int row = h_steps[s].row;
+            if (!copy_rows || row < 0) return 0;
+            hipEvent_t e = row_ev[row % 16];
+            MKH_HIP(hipEventRecord(e, stream));
+            MKH_HIP(hipStreamWaitEvent(copy_stream, e, 0));
+            MKH_HIP(hipMemcpyAsync(
+                dst_values + (size_t)(rows_done + row) * ncols,
+                logbuf.p + (size_t)row * ncols, (size_t)ncols * sizeof(R),
+                hipMemcpyDeviceToHost, copy_stream));
+            return 0;
+        };
         /* Launch */
         size_t nev = 0;
         const unsigned bt = (unsigned)cfg.block_threads;
@@ -792,16 +838,12 @@ struct Sim : SimBase {
                 int e = launch(prog->f_grid, grid, params, nev);
                 if (e) return e;
                 cur ^= 1;
+                e = emit_row(s);
+                if (e) return e;
             }
         } else if (driver == MKH_DRIVER_SHEET) {
             Args a = base_args();
-            MkSheetT<R> sh;
-            sh.edges = sh_edges.p;
-            sh.cnt = sh_cnt.p;
-            sh.err = sh_err.p;
-            sh.tiles_x = sh_tx;
-            sh.tiles_y = sh_ty;
-            sh.spin_limit = 1 << 22;
+            MkSheetT<R> sh = sheet_args();
             const unsigned ntiles = (unsigned)(sh_tx * sh_ty);
             long long s0 = 0;
             for (long long s = 0; s < ns; ++s) {
@@ -821,6 +863,8 @@ struct Sim : SimBase {
                     launches++;
                     sh_steps += i1 - i0;
                     s0 = s + 1;
+                    int e = emit_row(s);
+                    if (e) return e;
                 }
             }
         } else {
@@ -830,14 +874,17 @@ struct Sim : SimBase {
                                 ? (unsigned)((n + bt - 1) / bt)
                                 : (unsigned)ndom;
             Args a = base_args();
+            R* no_traj = nullptr;
             long long s0 = 0;
             for (long long s = 0; s < ns; ++s) {
                 if (h_steps[s].row >= 0 || s == ns - 1) {
                     int i0 = (int)s0, i1 = (int)(s + 1);
-                    void* params[] = {&a, &i0, &i1};
+                    void* params[] = {&a, &i0, &i1, &no_traj};
                     int e = launch(f, grid, params, nev);
                     if (e) return e;
                     s0 = s + 1;
+                    e = emit_row(s);
+                    if (e) return e;
                 }
             }
         }
@@ -847,6 +894,7 @@ struct Sim : SimBase {
         MKH_HIP(hipMemcpyAsync(&h, halt.p, sizeof(int), hipMemcpyDeviceToHost,
                                stream));
         MKH_HIP(hipStreamSynchronize(stream));
+        MKH_HIP(hipStreamSynchronize(copy_stream));
         if (driver == MKH_DRIVER_SHEET) {
             int e = 0;
             MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
@@ -859,10 +907,6 @@ struct Sim : SimBase {
         }
         long long valid = nr;
         if (h >= 0) valid = h + 1;
-        if (nr > 0 && ncols > 0 && dst_values)
-            MKH_HIP(hipMemcpy(dst_values + (size_t)rows_done * ncols,
-                              logbuf.p, (size_t)valid * ncols * sizeof(R),
-                              hipMemcpyDeviceToHost));
         for (long long k = 0; k < valid; ++k) {
             if (dst_times) dst_times[rows_done + k] = tlog[k];
             if (dst_paces) dst_paces[rows_done + k] = plog[k];
@@ -911,6 +955,8 @@ struct Sim : SimBase {
                                        (long long)i.size());
             }
             choose_driver();
+        } else if (k == "adj_release") {
+            if (v) adj.release();
         } else {
             return fail(MKH_E_ARG, "unknown option " + k);
         }
@@ -946,10 +992,46 @@ struct Sim : SimBase {
         return (unsigned)((n + bt - 1) / bt);
     }
 
+    /* Adjoint work buffers, kept between calls (sizes permitting) and
+     * released by the "adj_release" option or with the simulation. */
+    struct AdjBufs {
+        DevBuf<Step> steps;
+        DevBuf<R> ckpt, traj, work, mu[2], ab[2];
+        DevBuf<double> gfield, gx, gy, loss, target, trace;
+        DevBuf<int> probe_of;
+        DevBuf<long long> probes;
+        void release()
+        {
+            steps.release();
+            ckpt.release();
+            traj.release();
+            work.release();
+            for (int k = 0; k < 2; ++k) {
+                mu[k].release();
+                ab[k].release();
+            }
+            gfield.release();
+            gx.release();
+            gy.release();
+            loss.release();
+            target.release();
+            trace.release();
+            probe_of.release();
+            probes.release();
+        }
+    } adj;
+
+    /* Wall time of the last adj_run's phases (ms): forward with
+     * checkpoints, recomputation of earlier segments, reverse steps. */
+    double adj_ms[3] = {0, 0, 0};
+
     /* Discrete adjoint of a run from the current state (which it leaves
-     * unchanged). Forward with a checkpoint every K steps, then per segment
-     * (last first): recompute the segment's states into a trajectory buffer
-     * and take the reverse steps. */
+     * unchanged). The forward pass keeps a checkpoint at the start of every
+     * segment of K steps and writes the last segment's states straight into
+     * the trajectory buffer. The reverse pass then walks the segments from
+     * last to first: recompute the segment's states from its checkpoint
+     * (not needed for the last), then one fused reverse step per time step.
+     * With K >= N nothing is recomputed. */
     int adj_run(double tmin, double tmax, double dt, const double* events,
                 long long nev, const long long* probes, long long nprobe,
                 const double* target, long long nrows, int K, double* loss,
@@ -957,11 +1039,13 @@ struct Sim : SimBase {
     {
         /* trace_out != NULL: forward only, V at the probes after each step
          * into trace_out[s * nprobe + p] (nrows = its row capacity) */
-        if (!prog->f_adj_vjp)
+        if (!prog->f_adj_step)
             return fail(MKH_E_STATE, "kernel was generated without the "
                                      "adjoint");
-        if (cfg.coupling == MKH_COUPLING_GRAPH && driver != MKH_DRIVER_GRID)
-            return fail(MKH_E_STATE, "graph adjoint needs the grid layout");
+        if (cfg.coupling == MKH_COUPLING_GRAPH && driver != MKH_DRIVER_GRID &&
+            driver != MKH_DRIVER_BLOCK)
+            return fail(MKH_E_STATE, "graph adjoint needs the grid or block "
+                                     "layout");
         if (K < 1) return fail(MKH_E_ARG, "checkpoint interval must be >= 1");
         Planner pl;
         int f = pl.init(tmin, tmax, dt, events, nev, HUGE_VAL);
@@ -980,6 +1064,7 @@ struct Sim : SimBase {
         }
         const long long N = (long long)all.size();
         if (N == 0) return fail(MKH_E_ARG, "empty run");
+        if (N > INT_MAX) return fail(MKH_E_ARG, "run too long");
         if (nrows < N)
             return fail(MKH_E_ARG, std::string(trace_out ? "trace" : "target")
                                        + " needs one row per time step ("
@@ -988,99 +1073,167 @@ struct Sim : SimBase {
             if (probes[k] < 0 || probes[k] >= n)
                 return fail(MKH_E_ARG, "probe cell out of range");
         const size_t SN = (size_t)cfg.n_state * npad;
-        if (trace_out) K = (int)std::min<long long>(N, 1 << 30);
-        const long long nseg = (N + K - 1) / K;
-        DevBuf<Step> dsteps;
-        DevBuf<R> ckpt, traj, lamA, lamB, mu, abar;
-        DevBuf<double> gfield, gxa, gya, lossa, dtarget;
-        DevBuf<int> probe_of;
-        DevBuf<long long> dprobes;
-        DevBuf<double> dtrace;
+        if (trace_out) K = 1;
+        const long long Kl = std::min<long long>(K, N);
+        const long long nseg = (N + Kl - 1) / Kl;
+        const unsigned bt = (unsigned)cfg.block_threads;
+        const unsigned ggrid = grid_launch_size();
+        const unsigned cgrid = (unsigned)((n + bt - 1) / bt);
+        /* conductance-gradient partial sums: one per workgroup of the
+         * reverse kernel (per domain on the block layout) */
+        const size_t nparts =
+            std::max<size_t>(cgrid, driver == MKH_DRIVER_BLOCK
+                                        ? (size_t)std::max<long long>(1, ndom)
+                                        : 1);
+        DevBuf<Step>& dsteps = adj.steps;
+        DevBuf<R>& ckpt = adj.ckpt;
+        DevBuf<R>& traj = adj.traj;
+        DevBuf<R>& muA = adj.mu[0];
+        DevBuf<R>& muB = adj.mu[1];
+        DevBuf<R>& abA = adj.ab[0];
+        DevBuf<R>& abB = adj.ab[1];
+        DevBuf<double>& gfield = adj.gfield;
+        DevBuf<double>& gxa = adj.gx;
+        DevBuf<double>& gya = adj.gy;
+        DevBuf<double>& lossa = adj.loss;
+        DevBuf<double>& dtarget = adj.target;
+        DevBuf<double>& dtrace = adj.trace;
+        DevBuf<int>& probe_of = adj.probe_of;
+        DevBuf<long long>& dprobes = adj.probes;
         MKH_HIP(dsteps.alloc(N));
         MKH_HIP(hipMemcpy(dsteps.p, all.data(), N * sizeof(Step),
                           hipMemcpyHostToDevice));
-        MKH_HIP(ckpt.alloc(trace_out ? SN : nseg * SN));
-        MKH_HIP(traj.alloc((size_t)(trace_out ? 2 : K + 1) * SN));
-        MKH_HIP(lamA.alloc(SN));
-        MKH_HIP(lamB.alloc(SN));
-        MKH_HIP(mu.alloc(SN));
-        MKH_HIP(abar.alloc(npad));
-        MKH_HIP(gfield.alloc(std::max<size_t>(1, (size_t)cfg.n_field * npad)));
-        MKH_HIP(gxa.alloc(npad));
-        MKH_HIP(gya.alloc(npad));
-        MKH_HIP(lossa.alloc(std::max<long long>(1, nprobe)));
-        MKH_HIP(hipMemset(gfield.p, 0, gfield.n * sizeof(double)));
-        MKH_HIP(hipMemset(gxa.p, 0, npad * sizeof(double)));
-        MKH_HIP(hipMemset(gya.p, 0, npad * sizeof(double)));
-        MKH_HIP(hipMemset(lossa.p, 0, lossa.n * sizeof(double)));
-        MKH_HIP(hipMemset(abar.p, 0, npad * sizeof(R)));
-        std::vector<int> pof(npad, -1);
-        for (long long k = 0; k < nprobe; ++k) pof[probes[k]] = (int)k;
-        MKH_HIP(probe_of.alloc(npad));
-        MKH_HIP(hipMemcpy(probe_of.p, pof.data(), npad * sizeof(int),
-                          hipMemcpyHostToDevice));
+        /* trajectory: the longest segment plus its initial state (trace
+         * mode: two work buffers) */
+        const long long tslots = trace_out ? 2 : Kl + 1;
+        if (!trace_out && nseg > 1) MKH_HIP(ckpt.alloc((nseg - 1) * SN));
+        MKH_HIP(traj.alloc((size_t)tslots * SN));
         MKH_HIP(dprobes.alloc(std::max<long long>(1, nprobe)));
         if (nprobe > 0)
             MKH_HIP(hipMemcpy(dprobes.p, probes, nprobe * sizeof(long long),
                               hipMemcpyHostToDevice));
-        if (trace_out) MKH_HIP(dtrace.alloc(std::max<long long>(1, N * nprobe)));
-        MKH_HIP(dtarget.alloc(std::max<long long>(1, N * nprobe)));
-        if (nprobe > 0 && !trace_out)
-            MKH_HIP(hipMemcpy(dtarget.p, target, N * nprobe * sizeof(double),
+        MkAdjT<R> j;
+        std::memset(&j, 0, sizeof(j));
+        j.probes = dprobes.p;
+        j.n_probe = (int)nprobe;
+        j.target_row = -1;
+        if (trace_out) {
+            MKH_HIP(dtrace.alloc(std::max<long long>(1, N * nprobe)));
+            j.trace = dtrace.p;
+        } else {
+            MKH_HIP(muA.alloc(SN));
+            MKH_HIP(muB.alloc(SN));
+            MKH_HIP(abA.alloc(npad));
+            MKH_HIP(abB.alloc(npad));
+            MKH_HIP(gfield.alloc(
+                std::max<size_t>(1, (size_t)cfg.n_field * npad)));
+            MKH_HIP(gxa.alloc(nparts));
+            MKH_HIP(gya.alloc(nparts));
+            MKH_HIP(lossa.alloc(std::max<long long>(1, nprobe)));
+            MKH_HIP(hipMemset(gfield.p, 0, gfield.n * sizeof(double)));
+            MKH_HIP(hipMemset(gxa.p, 0, nparts * sizeof(double)));
+            MKH_HIP(hipMemset(gya.p, 0, nparts * sizeof(double)));
+            MKH_HIP(hipMemset(lossa.p, 0, lossa.n * sizeof(double)));
+            std::vector<int> pof(npad, -1);
+            for (long long k = 0; k < nprobe; ++k) pof[probes[k]] = (int)k;
+            MKH_HIP(probe_of.alloc(npad));
+            MKH_HIP(hipMemcpy(probe_of.p, pof.data(), npad * sizeof(int),
                               hipMemcpyHostToDevice));
+            MKH_HIP(dtarget.alloc(std::max<long long>(1, N * nprobe)));
+            if (nprobe > 0)
+                MKH_HIP(hipMemcpy(dtarget.p, target,
+                                  N * nprobe * sizeof(double),
+                                  hipMemcpyHostToDevice));
+            j.gfield = gfield.p;
+            j.gx_acc = gxa.p;
+            j.gy_acc = gya.p;
+            j.loss_acc = lossa.p;
+            j.probe_of = probe_of.p;
+            j.target = dtarget.p;
+        }
         int m1 = -1;
         MKH_HIP(hipMemcpy(halt.p, &m1, sizeof(int), hipMemcpyHostToDevice));
 
         Args a = base_args();
         a.steps = dsteps.p;
-        const unsigned bt = (unsigned)cfg.block_threads;
-        const unsigned ggrid = grid_launch_size();
-        const unsigned cgrid = (unsigned)((n + bt - 1) / bt);
-        auto grid_step = [&](R* src, R* dst, int si) -> int {
-            void* params[] = {&a, &src, &dst, &si};
+        auto grid_step = [&](R* src, R* dst, long long si) -> int {
+            int sii = (int)si;
+            void* params[] = {&a, &src, &dst, &sii};
             MKH_HIP(hipModuleLaunchKernel(prog->f_grid, ggrid, 1, 1, bt, 1,
                                           1, 0, stream, params, nullptr));
             return 0;
         };
-        MkAdjT<R> j;
-        std::memset(&j, 0, sizeof(j));
-        j.gfield = gfield.p;
-        j.gx_acc = gxa.p;
-        j.gy_acc = gya.p;
-        j.loss_acc = lossa.p;
-        j.probe_of = probe_of.p;
-        j.target = dtarget.p;
-        j.probes = dprobes.p;
-        j.trace = dtrace.p;
-        j.n_probe = (int)nprobe;
-        j.abar = abar.p;
-        const unsigned pgrid = (unsigned)((nprobe + 63) / 64);
-        /* Forward: two work buffers from the trajectory storage */
-        R* wa = traj.p;
-        R* wb = traj.p + SN;
+        auto slot = [&](long long k) { return traj.p + (size_t)k * SN; };
+        /* The persistent drivers (sheet, block, cells) advance a scratch
+         * copy of the state (they read and write state_a) and store each
+         * step's state into the trajectory when asked. The block layout
+         * also has a persistent reverse kernel. */
+        const bool use_pers = driver == MKH_DRIVER_SHEET ||
+                              driver == MKH_DRIVER_BLOCK ||
+                              driver == MKH_DRIVER_CELLS;
+        const bool block_rev = driver == MKH_DRIVER_BLOCK && !trace_out;
+        DevBuf<R>& work = adj.work;
+        if (use_pers) MKH_HIP(work.alloc(SN));
+        auto pers_run = [&](long long s0, long long s1, R* trajp) -> int {
+            Args as = a;
+            as.state_a = work.p;
+            int i0 = (int)s0, i1 = (int)s1;
+            if (driver == MKH_DRIVER_SHEET) {
+                MkSheetT<R> sh = sheet_args();
+                sh.g0 = (int)sh_steps;
+                sh.traj = trajp;
+                void* params[] = {&as, &sh, &i0, &i1};
+                MKH_HIP(hipModuleLaunchCooperativeKernel(
+                    prog->f_sheet, (unsigned)(sh_tx * sh_ty), 1, 1, 256, 1,
+                    1, 0, stream, params));
+                sh_steps += s1 - s0;
+            } else {
+                void* params[] = {&as, &i0, &i1, &trajp};
+                const bool blk = driver == MKH_DRIVER_BLOCK;
+                MKH_HIP(hipModuleLaunchKernel(
+                    blk ? prog->f_block : prog->f_cells,
+                    blk ? (unsigned)ndom : cgrid, 1, 1, bt, 1, 1, 0, stream,
+                    params, nullptr));
+            }
+            return 0;
+        };
+        hipEvent_t ev[4];
+        for (int k = 0; k < 4; ++k) MKH_HIP(hipEventCreate(&ev[k]));
+        struct EvFree {
+            hipEvent_t* e;
+            ~EvFree()
+            {
+                for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e[k]);
+            }
+        } evfree{ev};
+        MKH_HIP(hipEventRecord(ev[0], stream));
+
+        /* Forward */
+        R* wa = slot(0);
+        R* wb = slot(1);
         MKH_HIP(hipMemcpyAsync(wa, state[cur].p, SN * sizeof(R),
                                hipMemcpyDeviceToDevice, stream));
-        for (long long sg = 0; sg < nseg; ++sg) {
-            if (!trace_out)
-                MKH_HIP(hipMemcpyAsync(ckpt.p + sg * SN, wa, SN * sizeof(R),
+        if (trace_out) {
+            /* one step per launch, then gather the probes */
+            const unsigned pgrid = (unsigned)((nprobe + 63) / 64);
+            if (use_pers)
+                MKH_HIP(hipMemcpyAsync(work.p, state[cur].p, SN * sizeof(R),
                                        hipMemcpyDeviceToDevice, stream));
-            const long long s0 = sg * K, s1 = std::min(N, s0 + K);
-            for (long long si = s0; si < s1; ++si) {
-                int e = grid_step(wa, wb, (int)si);
+            for (long long si = 0; si < N; ++si) {
+                int e = use_pers ? pers_run(si, si + 1, nullptr)
+                                 : grid_step(wa, wb, si);
                 if (e) return e;
-                std::swap(wa, wb);
-                if (trace_out && nprobe > 0) {
+                if (!use_pers) std::swap(wa, wb);
+                if (nprobe > 0) {
                     MkAdjT<R> jt = j;
-                    jt.yn = wa;
+                    jt.yn = use_pers ? work.p : wa;
                     int sii = (int)si;
                     void* params[] = {&a, &jt, &sii};
-                    MKH_HIP(hipModuleLaunchKernel(prog->f_adj_probe, pgrid, 1,
-                                                  1, 64, 1, 1, 0, stream,
+                    MKH_HIP(hipModuleLaunchKernel(prog->f_adj_probe, pgrid,
+                                                  1, 1, 64, 1, 1, 0, stream,
                                                   params, nullptr));
                 }
             }
-        }
-        if (trace_out) {
             MKH_HIP(hipStreamSynchronize(stream));
             if (nprobe > 0)
                 MKH_HIP(hipMemcpy(trace_out, dtrace.p,
@@ -1088,51 +1241,137 @@ struct Sim : SimBase {
                                   hipMemcpyDeviceToHost));
             return 0;
         }
-        /* lam_N = dL/dy_N */
-        R* lam = lamA.p;
-        R* lam2 = lamB.p;
-        {
-            MkAdjT<R> j0 = j;
-            j0.yn = wa;
-            j0.mu = nullptr;
-            j0.lam_out = lam;
-            j0.target_row = (int)(N - 1);
-            int si = (int)N;
-            void* params[] = {&a, &j0, &si};
-            MKH_HIP(hipModuleLaunchKernel(prog->f_adj_couple, cgrid, 1, 1,
-                                          bt, 1, 1, 0, stream, params,
-                                          nullptr));
-        }
-        /* Backward over segments */
-        for (long long sg = nseg - 1; sg >= 0; --sg) {
-            const long long s0 = sg * K, s1 = std::min(N, s0 + K);
-            MKH_HIP(hipMemcpyAsync(traj.p, ckpt.p + sg * SN, SN * sizeof(R),
+        if (use_pers)
+            MKH_HIP(hipMemcpyAsync(work.p, state[cur].p, SN * sizeof(R),
                                    hipMemcpyDeviceToDevice, stream));
-            for (long long si = s0; si + 1 < s1; ++si) {
-                int e = grid_step(traj.p + (si - s0) * SN,
-                                  traj.p + (si - s0 + 1) * SN, (int)si);
-                if (e) return e;
+        for (long long sg = 0; use_pers && sg < nseg; ++sg) {
+            const long long s0 = sg * Kl, s1 = std::min(N, s0 + Kl);
+            R* keep = sg < nseg - 1 ? ckpt.p + sg * SN : slot(0);
+            MKH_HIP(hipMemcpyAsync(keep, work.p, SN * sizeof(R),
+                                   hipMemcpyDeviceToDevice, stream));
+            int e = pers_run(s0, s1, sg < nseg - 1 ? nullptr : slot(1));
+            if (e) return e;
+        }
+        for (long long sg = 0; !use_pers && sg < nseg; ++sg) {
+            const long long s0 = sg * Kl, s1 = std::min(N, s0 + Kl);
+            if (sg < nseg - 1) {
+                MKH_HIP(hipMemcpyAsync(ckpt.p + sg * SN, wa, SN * sizeof(R),
+                                       hipMemcpyDeviceToDevice, stream));
+                for (long long si = s0; si < s1; ++si) {
+                    int e = grid_step(wa, wb, si);
+                    if (e) return e;
+                    std::swap(wa, wb);
+                }
+            } else {
+                /* last segment: keep every state */
+                if (wa != slot(0))
+                    MKH_HIP(hipMemcpyAsync(slot(0), wa, SN * sizeof(R),
+                                           hipMemcpyDeviceToDevice, stream));
+                for (long long si = s0; si < s1; ++si) {
+                    int e = grid_step(slot(si - s0), slot(si - s0 + 1), si);
+                    if (e) return e;
+                }
+            }
+        }
+        MKH_HIP(hipEventRecord(ev[1], stream));
+
+        /* Reverse */
+        R* mu_in = nullptr;
+        R* ab_in = nullptr;
+        R* mu_out = muA.p;
+        R* ab_out = abA.p;
+        auto reverse_step = [&](const R* yn, const R* yn1, long long si,
+                                int row) -> int {
+            MkAdjT<R> js = j;
+            js.yn = yn;
+            js.yn1 = yn1;
+            js.mu_in = mu_in;
+            js.ab_in = ab_in;
+            js.mu_out = mu_out;
+            js.ab_out = ab_out;
+            js.target_row = row;
+            int sii = (int)si;
+            void* params[] = {&a, &js, &sii};
+            MKH_HIP(hipModuleLaunchKernel(prog->f_adj_step, cgrid, 1, 1, bt,
+                                          1, 1, 0, stream, params, nullptr));
+            mu_in = mu_out;
+            ab_in = ab_out;
+            mu_out = (mu_out == muA.p) ? muB.p : muA.p;
+            ab_out = (ab_out == abA.p) ? abB.p : abA.p;
+            return 0;
+        };
+        double recompute_ms = 0;
+        for (long long sg = nseg - 1; sg >= 0; --sg) {
+            const long long s0 = sg * Kl, s1 = std::min(N, s0 + Kl);
+            if (sg < nseg - 1) {
+                MKH_HIP(hipEventRecord(ev[2], stream));
+                MKH_HIP(hipMemcpyAsync(slot(0), ckpt.p + sg * SN,
+                                       SN * sizeof(R),
+                                       hipMemcpyDeviceToDevice, stream));
+                if (use_pers) {
+                    MKH_HIP(hipMemcpyAsync(work.p, ckpt.p + sg * SN,
+                                           SN * sizeof(R),
+                                           hipMemcpyDeviceToDevice, stream));
+                    int e = pers_run(s0, s1, slot(1));
+                    if (e) return e;
+                } else {
+                    for (long long si = s0; si < s1; ++si) {
+                        int e = grid_step(slot(si - s0), slot(si - s0 + 1),
+                                          si);
+                        if (e) return e;
+                    }
+                }
+                MKH_HIP(hipEventRecord(ev[3], stream));
+                MKH_HIP(hipEventSynchronize(ev[3]));
+                float ms = 0;
+                MKH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+                recompute_ms += ms;
+            }
+            if (block_rev) {
+                /* every reverse step of the segment in one launch; mu in
+                 * place in muA, abar carried in abA */
+                MkAdjT<R> js = j;
+                js.yn = slot(0);
+                js.mu_out = muA.p;
+                js.ab_out = abA.p;
+                int i0 = (int)s0, i1 = (int)s1;
+                int flags = (sg == nseg - 1 ? 1 : 0) | (sg == 0 ? 2 : 0);
+                void* params[] = {&a, &js, &i0, &i1, &flags};
+                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_block,
+                                              (unsigned)ndom, 1, 1, bt, 1, 1,
+                                              0, stream, params, nullptr));
+                continue;
             }
             for (long long si = s1 - 1; si >= s0; --si) {
-                MkAdjT<R> js = j;
-                js.yn = traj.p + (si - s0) * SN;
-                js.lam = lam;
-                js.mu = mu.p;
-                js.lam_out = lam2;
-                js.target_row = si >= 1 ? (int)(si - 1) : -1;
-                int sii = (int)si;
-                void* params[] = {&a, &js, &sii};
-                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_vjp, cgrid, 1, 1,
-                                              bt, 1, 1, 0, stream, params,
-                                              nullptr));
-                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_couple, cgrid, 1,
-                                              1, bt, 1, 1, 0, stream, params,
-                                              nullptr));
-                std::swap(lam, lam2);
+                int e = reverse_step(slot(si - s0), slot(si - s0 + 1), si,
+                                     (int)si);
+                if (e) return e;
             }
         }
-        MKH_HIP(hipStreamSynchronize(stream));
-        /* Reductions in double precision, in cell order */
+        /* closing pass: conductance terms of step 0 (y_0 is slot 0) */
+        if (!block_rev) {
+            int e = reverse_step(nullptr, slot(0), 0, -1);
+            if (e) return e;
+        }
+        MKH_HIP(hipEventRecord(ev[3], stream));
+        MKH_HIP(hipEventSynchronize(ev[3]));
+        if (use_pers && driver == MKH_DRIVER_SHEET) {
+            int e = 0;
+            MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
+                              hipMemcpyDeviceToHost));
+            if (e)
+                return fail(MKH_E_HIP, "sheet driver: a neighbour exchange "
+                                       "timed out");
+        }
+        {
+            float f0 = 0, f1 = 0;
+            MKH_HIP(hipEventElapsedTime(&f0, ev[0], ev[1]));
+            MKH_HIP(hipEventElapsedTime(&f1, ev[1], ev[3]));
+            adj_ms[0] = f0;
+            adj_ms[1] = recompute_ms;
+            adj_ms[2] = f1 - recompute_ms;
+        }
+        /* Reductions in double precision, in cell / workgroup order */
         std::vector<double> h(std::max<size_t>(npad, gfield.n));
         if (gfield_out && cfg.n_field > 0) {
             MKH_HIP(hipMemcpy(h.data(), gfield.p, gfield.n * sizeof(double),
@@ -1142,12 +1381,12 @@ struct Sim : SimBase {
                     gfield_out[k * n + c] = h[(size_t)k * npad + c];
         }
         double sx = 0, sy = 0, sl = 0;
-        MKH_HIP(hipMemcpy(h.data(), gxa.p, npad * sizeof(double),
+        MKH_HIP(hipMemcpy(h.data(), gxa.p, nparts * sizeof(double),
                           hipMemcpyDeviceToHost));
-        for (long long c = 0; c < n; ++c) sx += h[c];
-        MKH_HIP(hipMemcpy(h.data(), gya.p, npad * sizeof(double),
+        for (size_t b = 0; b < nparts; ++b) sx += h[b];
+        MKH_HIP(hipMemcpy(h.data(), gya.p, nparts * sizeof(double),
                           hipMemcpyDeviceToHost));
-        for (long long c = 0; c < n; ++c) sy += h[c];
+        for (size_t b = 0; b < nparts; ++b) sy += h[b];
         std::vector<double> hl(lossa.n);
         MKH_HIP(hipMemcpy(hl.data(), lossa.p, lossa.n * sizeof(double),
                           hipMemcpyDeviceToHost));
@@ -1157,6 +1396,12 @@ struct Sim : SimBase {
             gg_out[1] = sy;
         }
         if (loss) *loss = sl;
+        return 0;
+    }
+
+    int adj_timing(double* ms) override
+    {
+        for (int k = 0; k < 3; ++k) ms[k] = adj_ms[k];
         return 0;
     }
 
@@ -1278,13 +1523,13 @@ int mkh_program_build(const char* source, const char* options,
         MKH_HIP(hipModuleGetFunction(&prog->f_grid, prog->module, "mk_grid"));
         MKH_HIP(hipModuleGetFunction(&prog->f_sheet, prog->module, "mk_sheet"));
         /* present only in kernels generated with gradients */
-        if (hipModuleGetFunction(&prog->f_adj_vjp, prog->module,
-                                 "mk_adj_vjp") != hipSuccess ||
-            hipModuleGetFunction(&prog->f_adj_couple, prog->module,
-                                 "mk_adj_couple") != hipSuccess ||
+        if (hipModuleGetFunction(&prog->f_adj_step, prog->module,
+                                 "mk_adj_step") != hipSuccess ||
+            hipModuleGetFunction(&prog->f_adj_block, prog->module,
+                                 "mk_adj_block") != hipSuccess ||
             hipModuleGetFunction(&prog->f_adj_probe, prog->module,
                                  "mk_adj_probe") != hipSuccess) {
-            prog->f_adj_vjp = prog->f_adj_couple = prog->f_adj_probe =
+            prog->f_adj_step = prog->f_adj_block = prog->f_adj_probe =
                 nullptr;
             (void)hipGetLastError();
         }
@@ -1479,6 +1724,28 @@ int mkh_adj_run(void* sim, double tmin, double tmax, double dt,
     return SIM->adj_run(tmin, tmax, dt, events, n_events, probes, n_probes,
                         target, n_target_rows, checkpoint_every, loss,
                         grad_fields, grad_g, nullptr);
+}
+
+int mkh_host_alloc(long long bytes, void** ptr)
+{
+    if (!ptr || bytes < 0) return fail(MKH_E_ARG, "bad argument");
+    *ptr = nullptr;
+    if (bytes == 0) return 0;
+    MKH_HIP(hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault));
+    return 0;
+}
+
+int mkh_host_free(void* ptr)
+{
+    if (ptr) MKH_HIP(hipHostFree(ptr));
+    return 0;
+}
+
+int mkh_adj_timing(void* sim, double* ms3)
+{
+    CHECK_SIM();
+    if (!ms3) return fail(MKH_E_ARG, "null argument");
+    return SIM->adj_timing(ms3);
 }
 
 int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,

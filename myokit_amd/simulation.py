@@ -27,9 +27,10 @@ from . import _expressions as ex
 from . import _kernelgen
 from . import _native
 from ._datalog import (DOUBLE_PRECISION, LOG_BOUND, LOG_INTER, LOG_NONE,
-                       LOG_STATE, SINGLE_PRECISION, prepare_log, split_key)
-from ._err import (FindNanError, SimulationCancelledError, SimulationError,
-                   SimultaneousProtocolEventError)
+                       LOG_STATE, SINGLE_PRECISION, dimco, float_str,
+                       prepare_log, split_key)
+from ._err import (FindNanError, NumericalError, SimulationCancelledError,
+                   SimulationError, SimultaneousProtocolEventError)
 from ._model import Variable
 
 
@@ -862,7 +863,12 @@ class SimulationOpenCL:
         nrows = nrows.value
         dtype = np.float64 if self._precision == DOUBLE_PRECISION \
             else np.float32
-        values = np.empty((nrows, max(ncols, 1)), dtype=dtype)
+        shape = (nrows, max(ncols, 1))
+        if nrows * shape[1] * np.dtype(dtype).itemsize >= (1 << 20):
+            # page-locked: rows are copied while later steps compute
+            values = _native.pinned.empty(shape, dtype)
+        else:
+            values = np.empty(shape, dtype=dtype)
         times = np.zeros(nrows, dtype=np.float64)
         paces = np.zeros(nrows, dtype=np.float64)
         _native.check(L.mkh_sim_set_log(
@@ -945,7 +951,8 @@ class SimulationOpenCL:
         for k, var in enumerate(quantities):
             out[var.qname()] = values[:, k * n:(k + 1) * n].reshape(
                 (len(times),) + shape)
-        if report_nan and (err or not np.all(np.isfinite(values))):
+        # As DataLog.has_nan: a NaN in the final logged row
+        if report_nan and (err or (len(values) and np.isnan(values[-1]).any())):
             raise SimulationError(
                 'Numerical error found in simulation logs.')
         return out
@@ -970,38 +977,252 @@ class SimulationOpenCL:
                 log.extend_column(key, arr)
 
     def _report_nan(self, log):
+        """Raises SimulationError describing the first numerical error, as
+        ``_run`` does (``openclsim.py:1151-1281``)."""
+        fs = lambda x: float_str(x, precision=self._precision)  # noqa: E731
         txt = ['Numerical error found in simulation logs.']
         try:
-            t, icell, var, value = self._locate_nan(log)
-            txt.append('Encountered numerical error at t=' + repr(t)
-                       + ' in cell (' + ','.join(str(x) for x in icell)
-                       + ') when ' + var + '=' + repr(value) + '.')
+            time, icell, var, value, states, bounds, d = self.find_nan(
+                log, return_log=True)
+            icell_str = '(' + ','.join(str(x) for x in icell) + ')'
+            txt.append('Encountered numerical error at t=' + fs(time)
+                       + ' in cell ' + icell_str + ' when ' + var + '='
+                       + fs(value) + '.')
+            is_paced = self.is_paced(*icell)
+            is_paced_str = (', cell ' + icell_str + ' '
+                            + ('IS' if is_paced else 'is NOT') + ' paced')
+            vtime = self._model.time().qname()
+            vpace = self._model.binding('pace')
+            vpace = None if vpace is None else vpace.qname()
+            vdiff = None
+            if self._diffusion_enabled:
+                vdiff = self._model.binding('diffusion_current')
+                vdiff = None if vdiff is None else vdiff.qname()
+            neighbors = self.neighbors(*icell)
+            if len(self._dims) == 2:
+                neighbors_str = ', '.join(
+                    '(' + ','.join(str(i) for i in j) + ')' for j in neighbors)
+            else:
+                neighbors_str = ', '.join(str(j) for j in neighbors)
+
+            def variables(k, when):
+                txt.append('Simulation variables ' + when + ':')
+                txt.append('  Time: ' + fs(bounds[k][vtime]))
+                if vpace is not None:
+                    txt.append('  Pacing variable: ' + fs(bounds[k][vpace])
+                               + is_paced_str)
+                if vdiff is not None:
+                    txt.append('  Diffusion current: '
+                               + fs(bounds[k][vdiff]))
+                if neighbors:
+                    txt.append('  Connected cells: ' + neighbors_str)
+
+            txt.append('State during:')
+            txt.append(self._model.format_state(states[0],
+                                                precision=self._precision))
+            variables(0, 'during')
+            txt.append('Obtained ' + str(len(states))
+                       + ' previous state(s).')
+            if len(states) > 1:
+                bound = {'time': bounds[1][vtime], 'pace': 0,
+                         'diffusion_current': 0}
+                if is_paced and vpace is not None:
+                    bound['pace'] = bounds[1][vpace]
+                if vdiff is not None:
+                    bound['diffusion_current'] = bounds[1][vdiff]
+                eval_error = None
+                try:
+                    derivs = self._model.evaluate_derivatives(
+                        states[1], bound, self._precision,
+                        ignore_errors=False)
+                except NumericalError as ee:
+                    derivs = self._model.evaluate_derivatives(
+                        states[1], bound, self._precision,
+                        ignore_errors=True)
+                    eval_error = str(ee)
+                txt.append('State before:')
+                txt.append(self._model.format_state_derivatives(
+                    states[1], derivs, self._precision))
+                variables(1, 'before')
+                dt = d.time()
+                txt.append('Logged all variables for points: '
+                           + ', '.join(fs(t) for t in dt))
+                if len(dt) > 1:
+                    txt.append('Non-finite valued variables at t='
+                               + fs(dt[-2]))
+                    for key, values in d.items():
+                        if not np.isfinite(values[-2]):
+                            txt.append('  ' + str(key) + ' = '
+                                       + fs(values[-2]))
+                if eval_error is not None:
+                    txt.append('Error when evaluating derivatives:')
+                    txt.append(eval_error)
         except FindNanError as e:
             txt.append('Unable to pinpoint source of NaN, an error occurred:')
             txt.append(str(e))
         raise SimulationError('\n'.join(txt))
 
-    def _locate_nan(self, log):
-        """First logged point with a non-finite value: (time, cell index,
-        variable, value)."""
-        tkey = log.time_key()
-        if tkey is None:
-            raise FindNanError('Method requires a simulation log with time.')
-        first = None
-        for key, v in log.items():
-            a = np.asarray(v, dtype=float)
-            bad = np.nonzero(~np.isfinite(a))[0]
-            if len(bad) and (first is None or bad[0] < first[0]):
-                first = (bad[0], key, a[bad[0]])
-        if first is None:
-            raise FindNanError('No numerical error found in log.')
-        i, key, value = first
-        kdims, kname = split_key(key)
-        icell = tuple(int(x) for x in kdims[:-1].split('.')) if kdims else ()
-        return float(np.asarray(log[tkey])[i]), icell, kname, float(value)
-
     def find_nan(self, log, watch_var=None, safe_range=None,
                  return_log=False):
-        """Locates the first non-finite value in a log of this simulation:
-        returns ``(time, icell, var, value)``."""
-        return self._locate_nan(log)
+        """Searches for the origin of a bad value (NaN or inf) in a log of
+        this simulation (``SimulationOpenCL.find_nan``,
+        ``openclsim.py:395-702``).
+
+        The log must hold the state of every cell and all bound variables.
+        The first bad point (or, with ``watch_var`` and ``safe_range``, the
+        first time that state leaves the range) is found by bisection, then
+        the simulation is re-run from the logged point before it with a log
+        interval ten times smaller, until the interval drops below 0.5 and
+        the last re-run logs every step. Returns ``(time, icell, variable,
+        value, states, bounds)`` (plus a log of every variable over the last
+        points if ``return_log``): ``states[0]`` is the cell's state at the
+        error, ``states[1..]`` up to three earlier ones, ``bounds`` the
+        matching bound variables."""
+        need = [self._model.binding(x).qname() for x in self._global
+                if self._model.binding(x) is not None]
+        need = prepare_log(LOG_STATE + LOG_BOUND, self._model,
+                           dims=self._dims, global_vars=need)
+        for key in need:
+            if key not in log:
+                raise FindNanError(
+                    'Method requires a simulation log containing all states'
+                    ' and bound variables. Missing variable <' + key + '>.')
+
+        if watch_var is None:
+            def bisect(ar, lo, hi):
+                while True:
+                    if not np.isfinite(ar[lo]):
+                        return lo
+                    md = lo + int(np.ceil(0.5 * (hi - lo)))
+                    if md == hi:
+                        return hi
+                    if not np.isfinite(ar[md]):
+                        hi = md
+                    else:
+                        lo = md
+
+            def find_error_position(_log):
+                ifirst = kfirst = None
+                for key, ar in _log.items():
+                    ar = np.asarray(ar)
+                    if ifirst is None:
+                        if not np.isfinite(ar[-1]):
+                            kfirst = key
+                            ifirst = bisect(ar, 0, len(ar) - 1)
+                            if ifirst == 0:
+                                break
+                    elif not np.isfinite(ar[ifirst - 1]):
+                        kfirst = key
+                        ifirst = bisect(ar, 0, ifirst)
+                        if ifirst == 0:
+                            break
+                return ifirst, kfirst
+        else:
+            try:
+                watch_var = self._model.get(watch_var)
+            except KeyError:
+                raise FindNanError(
+                    'Variable <' + str(watch_var) + '> not found.')
+            if not watch_var.is_state():
+                raise FindNanError('The watched variable must be a state.')
+            try:
+                lo, hi = safe_range
+            except Exception:
+                raise FindNanError(
+                    'A safe range must be specified for the watched variable'
+                    ' as a tuple (lower, upper).')
+            if lo >= hi:
+                raise FindNanError(
+                    'The safe range must have a lower bound that is lower'
+                    ' than the upper bound.')
+
+            def find_error_position(_log):
+                ifirst = kfirst = None
+                post = '.' + watch_var.qname()
+                for dims in dimco(*self._dims):
+                    key = '.'.join(str(x) for x in dims) + post
+                    ar = np.asarray(_log[key])
+                    i = np.nonzero((ar < lo) | (ar > hi) | np.isnan(ar)
+                                   | np.isinf(ar))[0]
+                    if len(i):
+                        i = i[0]
+                        if ifirst is None or i < ifirst:
+                            kfirst, ifirst = key, i
+                        if i == 0:
+                            break
+                return ifirst, kfirst
+
+        time_var = self._model.time().qname()
+        states = list(self._model.states())
+
+        def rerun(start, duration, flags, interval):
+            state = []
+            for dims in dimco(*self._dims):
+                pre = '.'.join(str(x) for x in dims) + '.'
+                for s in states:
+                    state.append(float(log_[pre + s.qname()][start]))
+            old_state, old_time = self._state, self._time
+            self._state = state
+            self._time = float(log_[time_var][start])
+            try:
+                return self.run(duration, log=flags, log_interval=interval,
+                                report_nan=False)
+            finally:
+                self._state = old_state
+                self._time = old_time
+
+        log_ = log
+        t = np.asarray(log_[time_var])
+        dt = t[1] - t[0] if len(t) > 1 else 5
+        while dt > 0:
+            dt *= 0.1
+            if dt < 0.5:
+                dt = 0
+            ifirst, kfirst = find_error_position(log_)
+            if kfirst is None:
+                raise FindNanError('Error condition not found in log.')
+            if ifirst == 0:
+                raise FindNanError(
+                    'Unable to work with simulation logs where the error'
+                    ' condition is met in the very first data point.')
+            t = np.asarray(log_[time_var])
+            log_ = rerun(ifirst - 1, t[ifirst] - t[ifirst - 1],
+                         LOG_BOUND + LOG_STATE, dt)
+
+        ifirst, kfirst = find_error_position(log_)
+        if ifirst is None:
+            ifirst = 0
+            kfirst = next(iter(log_.keys()))
+        ndims = len(self._dims)
+        icell = [int(x) for x in kfirst.split('.')[0:ndims]]
+        pre = '.'.join(str(x) for x in icell) + '.'
+
+        def state_at(index):
+            st = [float(log_[pre + v.qname()][index]) for v in states]
+            b = {}
+            for var in self._model.variables(bound=True):
+                if var.binding() in self._global:
+                    b[var.qname()] = float(log_[var.qname()][index])
+                else:
+                    b[var.qname()] = float(log_[pre + var.qname()][index])
+            return st, b
+
+        times, out_states, bounds = [], [], []
+        for k in range(ifirst, ifirst - 4, -1):
+            if k < 0:
+                break
+            times.append(float(log_[time_var][k]))
+            st, b = state_at(k)
+            out_states.append(st)
+            bounds.append(b)
+        var = self._model.get('.'.join(kfirst.split('.')[ndims:]))
+        k = 1 if ifirst > 0 else 0
+        value = out_states[k][var.index()] if var.is_state() else \
+            bounds[k][var.qname()]
+        var = var.qname()
+        time = float(log_[time_var][ifirst])
+        if not return_log:
+            return time, icell, var, value, out_states, bounds
+        d = rerun(max(0, ifirst - 3), len(times) * self._step_size,
+                  LOG_BOUND + LOG_STATE + LOG_INTER, 0)
+        return time, icell, var, value, out_states, bounds, d

@@ -112,9 +112,10 @@ class Scenario:
                 s.set_field(name, values)
 
 
-def run_both(sc, duration, precision, var='membrane.V', log_interval=0.5):
-    """Runs the scenario on both engines; returns (product dict, oracle dict)
-    with 'time' and V as (rows, cells) arrays."""
+def run_product(sc, duration, precision, var='membrane.V',
+                log_interval=0.5):
+    """Runs the scenario on the GPU; returns a dict with 'time' and V as a
+    (rows, cells) array."""
     s = sc.product(precision)
     d = s.run(duration, log=['engine.time', var], log_interval=log_interval)
     n = s._ntotal
@@ -124,9 +125,16 @@ def run_both(sc, duration, precision, var='membrane.V', log_interval=0.5):
                 for c in range(n)]
     else:
         keys = [str(c) + '.' + var for c in range(n)]
-    P = {'time': np.asarray(d['engine.time'], dtype=float),
-         'V': np.stack([np.asarray(d[k], dtype=float) for k in keys], axis=1),
-         'state': np.array(s.state()), 'sim': s, 'log': d}
+    return {'time': np.asarray(d['engine.time'], dtype=float),
+            'V': np.stack([np.asarray(d[k], dtype=float) for k in keys],
+                          axis=1),
+            'state': np.array(s.state()), 'sim': s, 'log': d}
+
+
+def run_both(sc, duration, precision, var='membrane.V', log_interval=0.5):
+    """Runs the scenario on both engines; returns (product dict, oracle dict)
+    with 'time' and V as (rows, cells) arrays."""
+    P = run_product(sc, duration, precision, var, log_interval)
     o = sc.oracle()
     r = o.run(duration, log_vars=[var], log_interval=log_interval)
     O = {'time': r['time'], 'V': r[var], 'state': o.state, 'raw': r}

@@ -228,3 +228,74 @@ def test_torch_autograd_bridge():
     np.testing.assert_allclose(gna.grad.numpy(), g['ina.gNa'], rtol=1e-12)
     np.testing.assert_allclose(gca.grad.numpy(), g['ica.gCa'], rtol=1e-12)
     assert float(gx.grad) == pytest.approx(dgx, rel=1e-12)
+
+
+@pytest.mark.parametrize('K', [1000, 70])
+def test_adjoint_sheet_driver_matches_grid_fp64(K):
+    # 128 x 64 cells: large enough for the persistent sheet driver, whose
+    # forward and recomputation passes store the trajectory from registers.
+    # In double precision the sheet is bit-exact with the grid driver, so
+    # the gradients must be identical. K = 70 exercises recomputation.
+    out = []
+    for force_grid in (False, True):
+        m = myokit.load_model('example')
+        s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                    ncells=(128, 64), precision=64, rl=True)
+        s.set_step_size(0.01)
+        s.set_conductance(9.5, 6.0)
+        s.set_paced_cells(3, 64, 0, 0)
+        if force_grid:
+            s.set_engine_option('driver', 2)
+        adj = TissueAdjoint(s, [(10, 10), (60, 30), (127, 63)],
+                            checkpoint_every=K)
+        dur = 3.0
+        target = np.full((adj.n_steps(dur), 3), -40.0)
+        out.append(adj.loss_and_gradient(dur, target))
+        print('driver', s.engine_driver(), 'loss', out[-1][0], 'dgx',
+              out[-1][2], 'dgy', out[-1][3], flush=True)
+        assert s.engine_driver() == ('grid' if force_grid else 'sheet')
+    (l1, _, x1, y1), (l2, _, x2, y2) = out
+    assert l1 == l2 and x1 == x2 and y1 == y2
+
+
+@pytest.mark.parametrize('K', [10000, 37])
+def test_adjoint_block_layout_matches_grid_fp64(K):
+    # Cables and graph components run the adjoint on the block layout (one
+    # workgroup per domain, every reverse step of a segment per launch, abar
+    # carried between segments). Against the one-step-per-launch grid path:
+    # the same arithmetic per cell, only the conductance partial sums are
+    # added in another order.
+    def make(graph, force_grid):
+        m = myokit.load_model('example')
+        n = 60
+        s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                    ncells=n, precision=64)
+        s.set_step_size(0.005)
+        rng = np.random.default_rng(5)
+        if graph:
+            conns = [(i, i + 1, float(rng.uniform(5, 12)))
+                     for i in range(n - 1) if i != 29]
+            conns += [(3, 17, 7.0), (40, 55, 6.0)]
+            s.set_connections(conns)
+            s.set_paced_cell_list([0, 1, 30, 31])
+        else:
+            s.set_conductance(9.5)
+        s.set_field('ina.gNa', 16 * rng.uniform(0.8, 1.2, n))
+        if force_grid:
+            s.set_engine_option('driver', 2)
+        return s
+    for graph in (False, True):
+        res = []
+        for force_grid in (False, True):
+            s = make(graph, force_grid)
+            adj = TissueAdjoint(s, [5, 33, 58], checkpoint_every=K)
+            target = np.full((adj.n_steps(4.0), 3), -50.0)
+            res.append(adj.loss_and_gradient(4.0, target))
+            assert s.engine_driver() == ('grid' if force_grid else 'block')
+        (l1, g1, x1, _), (l2, g2, x2, _) = res
+        print('graph' if graph else 'cable', 'K', K, 'loss', l1, l2,
+              'dg', x1, x2, flush=True)
+        assert l1 == pytest.approx(l2, rel=1e-13)
+        assert x1 == pytest.approx(x2, rel=1e-11)
+        np.testing.assert_allclose(g1['ina.gNa'], g2['ina.gNa'], rtol=1e-11,
+                                   atol=1e-12 * np.max(np.abs(g2['ina.gNa'])))

@@ -1,0 +1,122 @@
+"""
+NaN handling on the GPU: the scenarios and expected answers of the
+reference's find_nan tests (``myokit/tests/test_simulation_opencl.py``,
+``SimulationOpenCLFindNanTest``, lines 1448-1589), restated.
+
+The voltage-clamped model: LR1991 with V demoted to a bound variable (the
+pacing level), Ca_i labelled as the membrane potential, V moved to the
+engine component. At V = -47.13 mV, ina.m's alpha is 0/0, so the cell
+clamped there by the protocol turns NaN at the first step after 1.234 ms.
+"""
+import numpy as np
+import pytest
+
+import myokit_amd as myokit
+
+pytestmark = pytest.mark.gpu
+
+
+def clamped_model():
+    m = myokit.load_model('example')
+    m.binding('pace').set_binding(None)
+    v = m.get('membrane.V')
+    v.set_rhs(-80)
+    v.demote()
+    v.set_label(None)
+    v.set_binding('pace')
+    m.get('ica.Ca_i').set_label('membrane_potential')
+    m.get('membrane').move_variable(v, m.get('engine'))
+    return m
+
+
+def protocol(t=1.234):
+    p = myokit.Protocol()
+    p.schedule(start=0, level=-80, duration=t)
+    p.schedule(start=t, level=-47.13, duration=1000)
+    return p
+
+
+@pytest.fixture(scope='module')
+def s2():
+    s = myokit.SimulationOpenCL(clamped_model(), protocol(), ncells=(3, 3))
+    s.set_paced_cell_list([[1, 1]])
+    return s
+
+
+def test_big_steps():
+    s = myokit.SimulationOpenCL(myokit.load_model('example'),
+                                myokit.pacing.blocktrain(1000, 2), ncells=10)
+    s.set_step_size(1)
+    with pytest.raises(myokit.SimulationError):
+        s.run(10)
+    for log in (['membrane.V'], myokit.LOG_STATE, myokit.LOG_BOUND):
+        s.reset()
+        with pytest.raises(myokit.SimulationError, match='Unable to pinpoint'):
+            s.run(10, log=log)
+
+
+def test_one_over_zero(s2):
+    s2.reset()
+    with pytest.raises(myokit.SimulationError, match='Time:  1.23') as e:
+        s2.run(5)
+    print(str(e.value)[:600], flush=True)
+
+
+def test_first_point(s2):
+    s2.reset()
+    x = s2.state(0, 0)
+    x[3] = float('nan')
+    s2.set_state(x, 2, 2)
+    with pytest.raises(myokit.SimulationError, match='met in the very first'):
+        s2.run(2)
+
+
+def test_without_error(s2):
+    try:
+        s2.set_protocol(None)
+        s2.reset()
+        d = s2.run(10)
+        with pytest.raises(myokit.FindNanError, match='not found in log'):
+            s2.find_nan(d)
+    finally:
+        s2.set_protocol(protocol())
+
+
+def test_manual_call(s2):
+    p = myokit.Protocol()
+    p.schedule(start=0, level=-90, duration=4)
+    p.schedule(start=4, level=-47.13, duration=1000)
+    try:
+        s2.set_protocol(p)
+        s2.reset()
+        x = s2.state(0, 0)
+        x[2] = 0.9      # lower j in one cell (makes the bisection branch)
+        s2.set_state(x, 2, 2)
+        d = s2.run(10, report_nan=False)
+        time, icell, variable, value, states, bounds = s2.find_nan(d)
+        assert time == pytest.approx(4.005, abs=1e-5)
+        assert icell == [1, 1]
+        assert variable == 'ina.m'
+        with pytest.raises(myokit.FindNanError, match='not found'):
+            s2.find_nan(d, 'x.y', [0, 1])
+        with pytest.raises(myokit.FindNanError, match='state'):
+            s2.find_nan(d, 'engine.time', [0, 1])
+        with pytest.raises(myokit.FindNanError, match='safe range'):
+            s2.find_nan(d, 'ina.m')
+        with pytest.raises(myokit.FindNanError, match='lower than'):
+            s2.find_nan(d, 'ina.m', [1, 0])
+        # Unpaced cells sit at V = 0, so ina inactivates quickly
+        time, icell, variable, value, states, bounds = s2.find_nan(
+            d, 'ina.j', [0.5, 1])
+        assert 2 < time < 3
+        assert icell == [2, 2]
+        assert variable == 'ina.j'
+        time, icell, variable, value, states, bounds = s2.find_nan(
+            d, 'ina.j', [0.2, 1])
+        assert 5 < time < 6
+        assert icell == [2, 2]
+        assert variable == 'ina.j'
+        assert len(states) == 4 and len(bounds) == 4
+        assert np.isfinite(states[1]).all()
+    finally:
+        s2.set_protocol(protocol())
