@@ -62,7 +62,7 @@ def cpu_baseline(nx, ny, sample_ms, threads):
     return r['steps'] * nx * ny / el, r['steps'], el
 
 
-def adjoint_leg(s, window_ms, checkpoint_every, reps):
+def adjoint_leg(s, window_ms, checkpoint_every, reps, vjp_precision=None):
     """Times loss_and_gradient (forward + discrete adjoint, gradients for gx
     and gy) over ``window_ms`` from the sheet's current state, 20 probe
     cells on the diagonal, constant target."""
@@ -73,7 +73,8 @@ def adjoint_leg(s, window_ms, checkpoint_every, reps):
     n_steps = None
     out = []
     for K in checkpoint_every:
-        adj = TissueAdjoint(s, probes, checkpoint_every=K or 1 << 30)
+        adj = TissueAdjoint(s, probes, checkpoint_every=K or 1 << 30,
+                            vjp_precision=vjp_precision)
         if n_steps is None:
             n_steps = adj.n_steps(window_ms)
         target = np.full((n_steps, len(probes)), -80.0)
@@ -180,11 +181,18 @@ def main():
             'window_ms': args.adjoint_ms, 'time_steps': n_adj,
             'fwd_ms': fwd_ms, 'fwd_adjoint_ms': t0 * 1e3,
             'checkpoint_every': K0, 'phases_ms': ph0,
+            'vjp_arithmetic': 'f64 (single-precision state)',
             'gradients': 'dL/dgx, dL/dgy; loss over 20 probe cells',
             'with_checkpoints_every_200': {
                 'ratio': res[1][1] * 1e3 / fwd_ms,
                 'fwd_adjoint_ms': res[1][1] * 1e3, 'phases_ms': res[1][2]},
         }
+        if args.precision == 32:
+            _, r32 = adjoint_leg(s, args.adjoint_ms, [0], args.adjoint_reps,
+                                 vjp_precision=32)
+            line['adjoint']['with_f32_vjp'] = {
+                'ratio': r32[0][1] * 1e3 / fwd_ms,
+                'fwd_adjoint_ms': r32[0][1] * 1e3, 'phases_ms': r32[0][2]}
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         cv, csteps, cel = cpu_baseline(args.nx, args.ny, args.cpu_sample_ms,

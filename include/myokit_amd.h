@@ -101,6 +101,12 @@ int mkh_sim_destroy(void* sim);
 int mkh_sim_set_state(void* sim, const double* state, long long n);
 int mkh_sim_get_state(void* sim, double* state, long long n);
 
+/* Copies the current device state of simulation src into dst, device to
+ * device (both of the same precision and geometry, e.g. engines of one
+ * model compiled for different log selections). No counterpart in the
+ * reference, whose state lives in host lists between runs. */
+int mkh_sim_copy_state(void* dst, void* src);
+
 /* Per-cell fields, cell-major / field-minor (openclsim.py:1082-1089). */
 int mkh_sim_set_fields(void* sim, const double* data, long long n);
 

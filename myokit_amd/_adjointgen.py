@@ -193,8 +193,88 @@ class _Rev:
         raise ValueError('No adjoint rule for ' + type(e).__name__)
 
 
-def generate_vjp_function(cm, precision):
-    """HIP source of ``mk_cell_vjp`` for a CellModel."""
+_VJP_EXP = """
+/* exp in double precision for the VJP (no bit-exact counterpart is needed
+ * here): x = k ln2 + r, |r| <= ln2 / 2, with the two-part ln2 of
+ * Cody and Waite applied by fused multiply-adds, exp(r) by its Taylor
+ * series to degree 11 (truncation below 2e-15 relative for |r| <= ln2/2),
+ * and 2^k by ldexp, which gives inf and 0 beyond the range. */
+__device__ __forceinline__ double mk_exp_vjp(double x)
+{
+    /* beyond these bounds the result is 0 or inf anyway; NaN passes */
+    x = x < -760.0 ? -760.0 : (x > 720.0 ? 720.0 : x);
+    const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+    const double r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56,
+                                   __builtin_fma(-k, 0x1.62e42fefa39efp-1,
+                                                 x));
+    double p = 0x1.ae64567f544e4p-26;                 /* 1/11! */
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);   /* 1/10! */
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);   /* 1/9!  */
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);   /* 1/8!  */
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);   /* 1/7!  */
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);   /* 1/6!  */
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);    /* 1/5!  */
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);    /* 1/4!  */
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);    /* 1/3!  */
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)k);
+}
+"""
+
+_MIXED_WRAPPER = """
+/* Single-precision state, double-precision VJP: the adjoint through an
+ * upstroke amplifies the VJP's own rounding by orders of magnitude (the
+ * quotient rule near removable singularities such as x / (1 - exp(-x))
+ * cancels in single precision), so the products are formed in double and
+ * only the results are rounded back. */
+__device__ __forceinline__ void mk_cell_vjp(
+    const Real* __restrict__ y, const Real t, const Real dt, const Real pace,
+    const Real idiff, const Real* __restrict__ fv,
+    const Real* __restrict__ lam, Real* __restrict__ mu,
+    Real* __restrict__ abar, Real* __restrict__ fbar)
+{
+    double yd[MK_NS], ld[MK_NS], md[MK_NS], fd[MK_NFA], fbd[MK_NFA], ad;
+#pragma unroll
+    for (int i = 0; i < MK_NS; ++i) {
+        yd[i] = (double)y[i];
+        ld[i] = (double)lam[i];
+    }
+#pragma unroll
+    for (int k = 0; k < MK_NFA; ++k) fd[k] = k < MK_NF ? (double)fv[k] : 0.0;
+    mk_cell_vjp_f64(yd, (double)t, (double)dt, (double)pace, (double)idiff,
+                    fd, ld, md, &ad, fbd);
+#pragma unroll
+    for (int i = 0; i < MK_NS; ++i) mu[i] = (Real)md[i];
+    *abar = (Real)ad;
+#pragma unroll
+    for (int k = 0; k < MK_NF; ++k) fbar[k] = (Real)fbd[k];
+}
+"""
+
+
+def generate_vjp_function(cm, precision, compute=None):
+    """HIP source of ``mk_cell_vjp`` for a CellModel. ``compute`` is the
+    precision of the VJP's arithmetic (default: ``precision``); double
+    arithmetic for a single-precision kernel gives a double-precision
+    ``mk_cell_vjp_f64`` and a wrapper that converts on entry and exit."""
+    import re
+    if compute is None or compute == precision:
+        return _vjp_source(cm, precision)
+    if not (precision != DOUBLE_PRECISION and compute == DOUBLE_PRECISION):
+        raise ValueError('Only a double-precision VJP for a single-precision'
+                         ' kernel can be mixed.')
+    src = _vjp_source(cm, DOUBLE_PRECISION)
+    src = re.sub(r'\bReal\b', 'double', src)
+    for name in ('mk_cell_vjp', 'mk_gmul', 'mk_gdiv'):
+        src = re.sub(r'\b' + name + r'\b', name + '_f64', src)
+    src = re.sub(r'\bmk_exp\(', 'mk_exp_vjp(', src)
+    return _VJP_EXP + src + _MIXED_WRAPPER
+
+
+def _vjp_source(cm, precision):
+    """HIP source of ``mk_cell_vjp`` computing in ``precision``."""
     r = _Rev(cm, precision)
     w = r.w
     tab = '    '

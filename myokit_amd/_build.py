@@ -41,7 +41,7 @@ def build_library(verbose=False):
 
 
 def check_kernel_compiles(precision=32, coupling=1, adjoint=False, rl=True,
-                          fields=()):
+                          fields=(), adjoint_precision=None):
     """Generates the LR1991 kernel (with the adjoint kernels if ``adjoint``)
     and compiles it for gfx950 with hiprtc (no device needed)."""
     import ctypes
@@ -52,7 +52,8 @@ def check_kernel_compiles(precision=32, coupling=1, adjoint=False, rl=True,
     cm = _prepare_cell_model(model, rl=rl, diffusion=True,
                              fields=list(fields), quantities=['membrane.V'])
     src = _kernelgen.generate_kernel(cm, precision, coupling,
-                                     adjoint=adjoint)
+                                     adjoint=adjoint,
+                                     adjoint_precision=adjoint_precision)
     prog = ctypes.c_void_p()
     log = ctypes.create_string_buffer(1 << 16)
     _native.check(_native.lib().mkh_program_build(

@@ -244,10 +244,30 @@ def prepare_log(log, model, dims=None, global_vars=None,
     names, or an existing log (checked for compatibility)."""
     dtype = np.float64 if precision == DOUBLE_PRECISION else np.float32
     dims = tuple(dims) if dims else ()
-    if dims:
-        dcombos = ['.'.join(str(i) for i in x) + '.' for x in dimco(*dims)]
-    else:
-        dcombos = ['']
+    _combos = []
+
+    def dcombos():
+        # Cell prefixes ('x.' or 'x.y.', first index fastest), built only
+        # when a name is expanded over every cell
+        if not _combos:
+            if dims:
+                _combos.extend(
+                    '.'.join(str(i) for i in x) + '.' for x in dimco(*dims))
+            else:
+                _combos.append('')
+        return _combos
+
+    def valid_index(kdims):
+        # 'x.' or 'x.y.' with every index in range
+        parts = kdims[:-1].split('.')
+        if len(parts) != len(dims):
+            return False
+        try:
+            idx = [int(x) for x in parts]
+        except ValueError:
+            return False
+        return all(0 <= i < n and str(i) == p
+                   for i, n, p in zip(idx, dims, parts))
     global_vars = list(global_vars or [])
     for name in global_vars:
         try:
@@ -293,7 +313,7 @@ def prepare_log(log, model, dims=None, global_vars=None,
             if not LOG_STATE & allowed_classes:
                 raise ValueError('This log does not support state variables.')
             for s in model.states():
-                for c in dcombos:
+                for c in dcombos():
                     log[c + s.qname()] = new()
             flag -= LOG_STATE
         if LOG_BOUND & flag:
@@ -304,7 +324,7 @@ def prepare_log(log, model, dims=None, global_vars=None,
                 if name in global_vars:
                     log[name] = new()
                 else:
-                    for c in dcombos:
+                    for c in dcombos():
                         log[c + name] = new()
             flag -= LOG_BOUND
         if LOG_INTER & flag:
@@ -316,14 +336,14 @@ def prepare_log(log, model, dims=None, global_vars=None,
                 if name in global_vars:
                     log[name] = new()
                 else:
-                    for c in dcombos:
+                    for c in dcombos():
                         log[c + name] = new()
             flag -= LOG_INTER
         if LOG_DERIV & flag:
             if not LOG_DERIV & allowed_classes:
                 raise ValueError('This log does not support time-derivatives.')
             for var in model.states():
-                for c in dcombos:
+                for c in dcombos():
                     log['dot(' + c + var.qname() + ')'] = new()
             flag -= LOG_DERIV
         if flag != 0:
@@ -361,7 +381,7 @@ def prepare_log(log, model, dims=None, global_vars=None,
                 if kname in global_vars:
                     raise ValueError('Cannot specify a cell index for global'
                                      f' logging variable <{kname}>.')
-                if kdims not in dcombos:
+                if not valid_index(kdims):
                     raise ValueError(f'Invalid index <{kdims}> in log.')
             elif dims:
                 if kname not in global_vars:
@@ -417,14 +437,14 @@ def prepare_log(log, model, dims=None, global_vars=None,
             if kname in global_vars:
                 raise ValueError('Cannot specify a cell index for global'
                                  f' logging variable <{kname}>.')
-            if kdims not in dcombos:
+            if not valid_index(kdims):
                 raise ValueError(f'Invalid index <{kdims}> in list.')
             k = kdims + kname
             log['dot(' + k + ')' if deriv else k] = new()
         elif kname in global_vars:
             log['dot(' + kname + ')' if deriv else kname] = new()
         else:
-            for c in dcombos:
+            for c in dcombos():
                 k = c + kname
                 log['dot(' + k + ')' if deriv else k] = new()
     t = model.time()

@@ -513,12 +513,29 @@ def sheet_enabled(precision):
     return True
 
 
+SHEET_CPT_CHOICES = (1, 2, 4, 8)
+
+
 def generate_kernel(cm, precision, coupling, block_threads=256,
-                    native_maths=False, adjoint=False):
-    """Returns the complete HIP source for hiprtc."""
+                    native_maths=False, adjoint=False, tile_cpt=None,
+                    tile_waves=None, adjoint_precision=None):
+    """Returns the complete HIP source for hiprtc. ``tile_cpt`` (cells
+    per thread of the persistent sheet: tile height 4 x cpt) and
+    ``tile_waves`` (waves per SIMD its register budget allows) default to
+    :func:`sheet_cpt` and :func:`sheet_waves`. With ``adjoint``, the
+    adjoint kernels are appended; ``adjoint_precision`` is the precision of
+    the cell VJP's arithmetic (default: ``precision``)."""
     if block_threads % 64 or not 64 <= block_threads <= 1024:
         raise ValueError('block_threads must be a multiple of 64 in'
                          ' [64, 1024].')
+    cpt = sheet_cpt(precision) if tile_cpt is None else int(tile_cpt)
+    waves = sheet_waves(precision) if tile_waves is None \
+        else int(tile_waves)
+    if cpt not in SHEET_CPT_CHOICES:
+        raise ValueError('sheet_cpt must be one of '
+                         + str(SHEET_CPT_CHOICES) + '.')
+    if not 1 <= waves <= 8:
+        raise ValueError('sheet_waves must be in [1, 8].')
     vm = cm.model.label('membrane_potential')
     i_vm = vm.index() if (vm is not None and vm.is_state()) else 0
     bcells = 2048 if precision == DOUBLE_PRECISION else 4096
@@ -533,8 +550,8 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_COUPLING ' + str(int(coupling)),
         '#define MK_BT ' + str(int(block_threads)),
         '#define MK_BCELLS ' + str(bcells),
-        '#define MK_SCPT ' + str(sheet_cpt(precision)),
-        '#define MK_SWAVES ' + str(sheet_waves(precision)),
+        '#define MK_SCPT ' + str(cpt),
+        '#define MK_SWAVES ' + str(waves),
         'typedef ' + real + ' Real;',
         _read('mk_math.h'),
         _read('mk_args.h'),
@@ -544,19 +561,23 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
     ]
     if adjoint:
         from ._adjointgen import generate_vjp_function
-        parts.append(generate_vjp_function(cm, precision))
+        parts.append(generate_vjp_function(cm, precision, adjoint_precision))
         parts.append(_read('mk_adjoint.hip'))
     return '\n'.join(parts)
 
 
 def compile_options(precision):
     """hiprtc options. Contraction is disabled in double precision so that
-    results match the CPU oracle bit for bit; single precision allows fused
-    multiply-adds. Packed FP32 instructions are disabled."""
+    results match the CPU oracle bit for bit. Single precision fuses a
+    multiply and an add only within one source expression
+    (``-ffp-contract=on``): with ``fast`` the compiler also fuses across
+    statements, differently in each kernel, and the drivers (and runs split
+    into several launches) stop giving identical results. Packed FP32
+    instructions are disabled."""
     opts = ['-O3', '-std=c++17',
             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
     if precision == DOUBLE_PRECISION:
         opts.append('-ffp-contract=off')
     else:
-        opts.append('-ffp-contract=fast')
+        opts.append('-ffp-contract=on')
     return ' '.join(opts)

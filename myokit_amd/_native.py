@@ -83,6 +83,7 @@ SIGNATURES = {
     'mkh_adj_run': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll, _i,
                          _pd, _pd, _pd]),
     'mkh_adj_timing': (_i, [_vp, _pd]),
+    'mkh_sim_copy_state': (_i, [_vp, _vp]),
     'mkh_host_alloc': (_i, [_ll, ctypes.POINTER(ctypes.c_void_p)]),
     'mkh_host_free': (_i, [_vp]),
     'mkh_adj_trace': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll]),

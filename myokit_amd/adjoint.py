@@ -30,8 +30,21 @@ from ._err import SimultaneousProtocolEventError
 class TissueAdjoint:
     """Discrete-adjoint gradients for a configured SimulationOpenCL."""
 
-    def __init__(self, sim, probes, checkpoint_every=200):
+    def __init__(self, sim, probes, checkpoint_every=200,
+                 vjp_precision=None):
         self.sim = sim
+        # Double-precision VJP arithmetic by default: in single precision
+        # the adjoint through an upstroke amplifies the VJP's rounding
+        # (cancellation near removable singularities) by orders of
+        # magnitude. vjp_precision=32 trades that accuracy for speed.
+        from ._datalog import DOUBLE_PRECISION, SINGLE_PRECISION
+        if vjp_precision is None:
+            vjp_precision = DOUBLE_PRECISION
+        if vjp_precision not in (SINGLE_PRECISION, DOUBLE_PRECISION):
+            raise ValueError('vjp_precision must be 32 or 64.')
+        if sim._precision == DOUBLE_PRECISION:
+            vjp_precision = DOUBLE_PRECISION
+        self.vjp_precision = vjp_precision
         p = np.asarray(probes)
         if p.ndim == 2:
             # (x, y) pairs in 2D
@@ -65,8 +78,10 @@ class TissueAdjoint:
 
     def _engine(self):
         sim = self.sim
-        eng, cm = sim._get_engine([], adjoint=True)
+        eng, cm = sim._get_engine([], adjoint=True,
+                                  adjoint_precision=self.vjp_precision)
         sim._configure(eng)
+        self._eng = eng
         return eng
 
     def _check(self, code, what):
@@ -106,7 +121,7 @@ class TissueAdjoint:
         """Wall time in ms of the last ``loss_and_gradient``'s phases:
         ``{'forward', 'recompute', 'reverse'}``."""
         ms = np.zeros(3)
-        eng = self.sim._engine
+        eng = getattr(self, '_eng', None)
         if eng is None:
             return {'forward': 0.0, 'recompute': 0.0, 'reverse': 0.0}
         _native.check(_native.lib().mkh_adj_timing(
@@ -116,7 +131,7 @@ class TissueAdjoint:
     def release(self):
         """Frees the device trajectory and work buffers kept between
         calls."""
-        eng = self.sim._engine
+        eng = getattr(self, '_eng', None)
         if eng is not None:
             _native.check(_native.lib().mkh_sim_set_option(
                 eng.h, b'adj_release', 1), 'set_option')

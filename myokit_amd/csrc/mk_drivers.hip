@@ -527,13 +527,16 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
         }
         cur ^= 1;
     }
+    /* The final state goes to the other buffer: a neighbour tile may still
+     * be reading its launch-start halo from state_a (nothing orders the
+     * two in a one-step launch). The host swaps the buffers. */
 #pragma unroll
     for (int k = 0; k < MK_SCPT; ++k) {
         if (act[k]) {
             const long long c = ix + (gy0 + w * MK_SCPT + k) * nx;
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i)
-                a.state_a[(long long)i * a.npad + c] = y[k][i];
+                a.state_b[(long long)i * a.npad + c] = y[k][i];
         }
     }
     /* The final step count is published with the state at kernel end */

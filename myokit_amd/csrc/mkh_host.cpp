@@ -220,6 +220,7 @@ struct SimBase {
                         const long long*, long long, const double*, long long,
                         int, double*, double*, double*, double*) = 0;
     virtual int adj_timing(double*) = 0;
+    virtual int copy_state_from(SimBase* src) = 0;
 };
 
 template <typename R>
@@ -345,9 +346,31 @@ struct Sim : SimBase {
         return 0;
     }
 
+    /* Whether driver d can run this geometry and coupling: the block
+     * driver needs every domain in one workgroup's LDS, the sheet a 2D
+     * homogeneous stencil whose tiles are co-resident, the cells driver
+     * uncoupled cells. */
+    bool feasible(int d) const
+    {
+        switch (d) {
+        case MKH_DRIVER_CELLS:
+            return cfg.coupling == MKH_COUPLING_NONE;
+        case MKH_DRIVER_BLOCK:
+            return max_domain <= max_block_cells;
+        case MKH_DRIVER_GRID:
+            return true;
+        case MKH_DRIVER_SHEET:
+            return cfg.coupling == MKH_COUPLING_STENCIL && cfg.ny > 1 &&
+                   sh_tx > 0 && sh_ty > 0 && sh_cnt.p &&
+                   (long long)sh_tx * sh_ty <= sh_capacity;
+        default:
+            return false;
+        }
+    }
+
     void choose_driver()
     {
-        if (forced_driver >= 0) {
+        if (forced_driver >= 0 && feasible(forced_driver)) {
             driver = forced_driver;
             return;
         }
@@ -375,6 +398,22 @@ struct Sim : SimBase {
                 soa[(size_t)i * npad + c] = (R)s[c * cfg.n_state + i];
         MKH_HIP(hipMemcpy(state[cur].p, soa.data(), soa.size() * sizeof(R),
                           hipMemcpyHostToDevice));
+        return 0;
+    }
+
+    /* Device-to-device copy of another simulation's current state (same
+     * precision, state count and cell count, so the same SoA layout). */
+    int copy_state_from(SimBase* src_base) override
+    {
+        Sim<R>* src = dynamic_cast<Sim<R>*>(src_base);
+        if (!src || src->cfg.n_state != cfg.n_state || src->n != n ||
+            src->npad != npad)
+            return fail(MKH_E_ARG, "states of different layouts");
+        MKH_HIP(hipStreamSynchronize(src->stream));
+        MKH_HIP(hipMemcpyAsync(state[cur].p, src->state[src->cur].p,
+                               (size_t)cfg.n_state * npad * sizeof(R),
+                               hipMemcpyDeviceToDevice, stream));
+        MKH_HIP(hipStreamSynchronize(stream));
         return 0;
     }
 
@@ -474,7 +513,9 @@ struct Sim : SimBase {
     void choose_driver_graph()
     {
         ndom = 0;   /* set by build_graph_layout */
-        if (forced_driver >= 0) {
+        if (forced_driver == MKH_DRIVER_GRID ||
+            (forced_driver == MKH_DRIVER_BLOCK &&
+             feasible(MKH_DRIVER_BLOCK))) {
             driver = forced_driver;
             return;
         }
@@ -842,12 +883,13 @@ struct Sim : SimBase {
                 if (e) return e;
             }
         } else if (driver == MKH_DRIVER_SHEET) {
-            Args a = base_args();
             MkSheetT<R> sh = sheet_args();
             const unsigned ntiles = (unsigned)(sh_tx * sh_ty);
             long long s0 = 0;
             for (long long s = 0; s < ns; ++s) {
                 if (h_steps[s].row >= 0 || s == ns - 1) {
+                    /* reads state[cur], writes state[cur ^ 1] */
+                    Args a = base_args();
                     int i0 = (int)s0, i1 = (int)(s + 1);
                     sh.g0 = (int)sh_steps;
                     void* params[] = {&a, &sh, &i0, &i1};
@@ -862,6 +904,7 @@ struct Sim : SimBase {
                     }
                     launches++;
                     sh_steps += i1 - i0;
+                    cur ^= 1;
                     s0 = s + 1;
                     int e = emit_row(s);
                     if (e) return e;
@@ -947,6 +990,13 @@ struct Sim : SimBase {
         } else if (k == "time_kernels") {
             time_kernels = v != 0;
         } else if (k == "driver") {
+            if (v < -1 || v > MKH_DRIVER_SHEET ||
+                (v >= 0 && !feasible((int)v)) ||
+                (have_graph && v >= 0 && v != MKH_DRIVER_GRID &&
+                 v != MKH_DRIVER_BLOCK))
+                return fail(MKH_E_ARG, "driver " + std::to_string(v) +
+                                           " cannot run this geometry and "
+                                           "coupling");
             forced_driver = (int)v;
             if (have_graph) {
                 std::vector<long long> i(ci), jj(cj);
@@ -996,7 +1046,7 @@ struct Sim : SimBase {
      * released by the "adj_release" option or with the simulation. */
     struct AdjBufs {
         DevBuf<Step> steps;
-        DevBuf<R> ckpt, traj, work, mu[2], ab[2];
+        DevBuf<R> ckpt, traj, work, work2, mu[2], ab[2];
         DevBuf<double> gfield, gx, gy, loss, target, trace;
         DevBuf<int> probe_of;
         DevBuf<long long> probes;
@@ -1006,6 +1056,7 @@ struct Sim : SimBase {
             ckpt.release();
             traj.release();
             work.release();
+            work2.release();
             for (int k = 0; k < 2; ++k) {
                 mu[k].release();
                 ab[k].release();
@@ -1173,12 +1224,20 @@ struct Sim : SimBase {
                               driver == MKH_DRIVER_CELLS;
         const bool block_rev = driver == MKH_DRIVER_BLOCK && !trace_out;
         DevBuf<R>& work = adj.work;
+        DevBuf<R>& work2 = adj.work2;
         if (use_pers) MKH_HIP(work.alloc(SN));
+        if (use_pers && driver == MKH_DRIVER_SHEET)
+            MKH_HIP(work2.alloc(SN));
+        /* wcur: the scratch state the persistent drivers advance (the sheet
+         * writes its result to the other buffer) */
+        R* wcur = work.p;
         auto pers_run = [&](long long s0, long long s1, R* trajp) -> int {
             Args as = a;
-            as.state_a = work.p;
+            as.state_a = wcur;
             int i0 = (int)s0, i1 = (int)s1;
             if (driver == MKH_DRIVER_SHEET) {
+                R* wnext = wcur == work.p ? work2.p : work.p;
+                as.state_b = wnext;
                 MkSheetT<R> sh = sheet_args();
                 sh.g0 = (int)sh_steps;
                 sh.traj = trajp;
@@ -1187,6 +1246,7 @@ struct Sim : SimBase {
                     prog->f_sheet, (unsigned)(sh_tx * sh_ty), 1, 1, 256, 1,
                     1, 0, stream, params));
                 sh_steps += s1 - s0;
+                wcur = wnext;
             } else {
                 void* params[] = {&as, &i0, &i1, &trajp};
                 const bool blk = driver == MKH_DRIVER_BLOCK;
@@ -1217,7 +1277,7 @@ struct Sim : SimBase {
             /* one step per launch, then gather the probes */
             const unsigned pgrid = (unsigned)((nprobe + 63) / 64);
             if (use_pers)
-                MKH_HIP(hipMemcpyAsync(work.p, state[cur].p, SN * sizeof(R),
+                MKH_HIP(hipMemcpyAsync(wcur, state[cur].p, SN * sizeof(R),
                                        hipMemcpyDeviceToDevice, stream));
             for (long long si = 0; si < N; ++si) {
                 int e = use_pers ? pers_run(si, si + 1, nullptr)
@@ -1226,7 +1286,7 @@ struct Sim : SimBase {
                 if (!use_pers) std::swap(wa, wb);
                 if (nprobe > 0) {
                     MkAdjT<R> jt = j;
-                    jt.yn = use_pers ? work.p : wa;
+                    jt.yn = use_pers ? wcur : wa;
                     int sii = (int)si;
                     void* params[] = {&a, &jt, &sii};
                     MKH_HIP(hipModuleLaunchKernel(prog->f_adj_probe, pgrid,
@@ -1242,12 +1302,12 @@ struct Sim : SimBase {
             return 0;
         }
         if (use_pers)
-            MKH_HIP(hipMemcpyAsync(work.p, state[cur].p, SN * sizeof(R),
+            MKH_HIP(hipMemcpyAsync(wcur, state[cur].p, SN * sizeof(R),
                                    hipMemcpyDeviceToDevice, stream));
         for (long long sg = 0; use_pers && sg < nseg; ++sg) {
             const long long s0 = sg * Kl, s1 = std::min(N, s0 + Kl);
             R* keep = sg < nseg - 1 ? ckpt.p + sg * SN : slot(0);
-            MKH_HIP(hipMemcpyAsync(keep, work.p, SN * sizeof(R),
+            MKH_HIP(hipMemcpyAsync(keep, wcur, SN * sizeof(R),
                                    hipMemcpyDeviceToDevice, stream));
             int e = pers_run(s0, s1, sg < nseg - 1 ? nullptr : slot(1));
             if (e) return e;
@@ -1309,7 +1369,7 @@ struct Sim : SimBase {
                                        SN * sizeof(R),
                                        hipMemcpyDeviceToDevice, stream));
                 if (use_pers) {
-                    MKH_HIP(hipMemcpyAsync(work.p, ckpt.p + sg * SN,
+                    MKH_HIP(hipMemcpyAsync(wcur, ckpt.p + sg * SN,
                                            SN * sizeof(R),
                                            hipMemcpyDeviceToDevice, stream));
                     int e = pers_run(s0, s1, slot(1));
@@ -1599,6 +1659,13 @@ int mkh_sim_get_state(void* sim, double* s, long long n)
 {
     CHECK_SIM();
     return SIM->get_state(s, n);
+}
+
+int mkh_sim_copy_state(void* dst, void* src)
+{
+    if (!dst || !src) return fail(MKH_E_ARG, "null simulation");
+    return static_cast<SimBase*>(dst)->copy_state_from(
+        static_cast<SimBase*>(src));
 }
 
 int mkh_sim_set_fields(void* sim, const double* d, long long n)

@@ -169,6 +169,7 @@ class SimulationOpenCL:
         self._graph_version = 0
         self._dev_newer = False
         self._engine = None
+        self._engines = OrderedDict()   # kernel key -> engine (recent last)
         if not model.is_valid():
             model.validate()
         if model.has_interdependent_components():
@@ -239,6 +240,7 @@ class SimulationOpenCL:
         self._engine_key = None
         self._device = 0
         self._block_threads = 256
+        self._tuning = {}
         self._options = {}
         self.last_run_stats = {}
 
@@ -621,6 +623,49 @@ class SimulationOpenCL:
             return _native.COUPLING_HETERO
         return _native.COUPLING_STENCIL
 
+    def set_tuning(self, driver=None, block_threads=None, sheet_cpt=None,
+                   sheet_waves=None):
+        """Launch configuration of the device engine (chosen automatically
+        by default; :func:`myokit_amd.tune.autotune` measures and picks
+        one). ``driver``: ``'auto'``, ``'cells'``, ``'block'``, ``'grid'``
+        or ``'sheet'``; ``block_threads``: threads per workgroup of the
+        cells, block and grid kernels (multiple of 64); ``sheet_cpt``:
+        cells per thread of the persistent sheet (tile 64 x 4*cpt);
+        ``sheet_waves``: waves per SIMD its register budget allows. Arguments
+        left as None keep their current value."""
+        names = {'auto': -1, 'cells': _native.DRIVER_CELLS,
+                 'block': _native.DRIVER_BLOCK, 'grid': _native.DRIVER_GRID,
+                 'sheet': _native.DRIVER_SHEET}
+        if block_threads is not None:
+            block_threads = int(block_threads)
+            if block_threads % 64 or not 64 <= block_threads <= 1024:
+                raise ValueError('block_threads must be a multiple of 64 in'
+                                 ' [64, 1024].')
+            self._block_threads = block_threads
+        if sheet_cpt is not None:
+            if int(sheet_cpt) not in _kernelgen.SHEET_CPT_CHOICES:
+                raise ValueError('sheet_cpt must be one of '
+                                 + str(_kernelgen.SHEET_CPT_CHOICES) + '.')
+            self._tuning['sheet_cpt'] = int(sheet_cpt)
+        if sheet_waves is not None:
+            if not 1 <= int(sheet_waves) <= 8:
+                raise ValueError('sheet_waves must be in [1, 8].')
+            self._tuning['sheet_waves'] = int(sheet_waves)
+        if driver is not None:
+            if driver not in names:
+                raise ValueError('Unknown driver ' + repr(driver) + '.')
+            self._tuning['driver'] = driver
+            self.set_engine_option('driver', names[driver])
+
+    def tuning(self):
+        """The current launch configuration (see :meth:`set_tuning`)."""
+        return {'driver': self._tuning.get('driver', 'auto'),
+                'block_threads': self._block_threads,
+                'sheet_cpt': self._tuning.get(
+                    'sheet_cpt', _kernelgen.sheet_cpt(self._precision)),
+                'sheet_waves': self._tuning.get(
+                    'sheet_waves', _kernelgen.sheet_waves(self._precision))}
+
     def set_engine_option(self, key, value):
         """Tuning option passed to the device library (``batch_steps``,
         ``log_rows``, ``time_kernels``, ``driver``)."""
@@ -628,6 +673,7 @@ class SimulationOpenCL:
         if self._engine is not None:
             _native.check(_native.lib().mkh_sim_set_option(
                 self._engine.h, key.encode(), int(value)), 'set_option')
+            self._engine.options[key] = int(value)
 
     def engine_driver(self):
         """Name of the device execution driver of the last run (``cells``,
@@ -656,23 +702,57 @@ class SimulationOpenCL:
         return {'kernel_ms': ms.value, 'launches': nl.value,
                 'cell_steps': cs.value}
 
-    def _get_engine(self, quantities, adjoint=False):
+    def _get_engine(self, quantities, adjoint=False, adjoint_precision=None):
         fields = list(self._fields.keys())
         cm = _kernelgen.CellModel(self._model, self._bound_variables,
                                   self._rl_states, fields, quantities)
         coupling = self._coupling()
+        tune = self._tuning
         src = _kernelgen.generate_kernel(
             cm, self._precision, coupling, self._block_threads,
-            self._native_math, adjoint=adjoint)
+            self._native_math, adjoint=adjoint,
+            adjoint_precision=adjoint_precision,
+            tile_cpt=tune.get('sheet_cpt'),
+            tile_waves=tune.get('sheet_waves'))
         opts = _kernelgen.compile_options(self._precision)
         key = (src, opts, self._ntotal, self._dims)
         if self._engine is not None and self._engine_key == key:
             return self._engine, cm
+        eng = self._engines.get(key)
+        if eng is None:
+            eng = self._new_engine(src, opts, cm, coupling, fields,
+                                   quantities)
+        else:
+            self._engines.move_to_end(key)
+        for k, v in self._options.items():
+            if eng.options.get(k) != v:
+                _native.check(_native.lib().mkh_sim_set_option(
+                    eng.h, k.encode(), v), 'set_option')
+                eng.options[k] = v
+        old = self._engine
+        if old is not None and self._dev_newer:
+            # the latest state moves across on the device
+            _native.check(_native.lib().mkh_sim_copy_state(eng.h, old.h),
+                          'mkh_sim_copy_state')
+            self._host_dirty = False
+        else:
+            self._host_dirty = True
+        self._engines[key] = eng
+        while len(self._engines) > 3:
+            k0 = next(iter(self._engines))
+            if self._engines[k0] is eng:
+                break
+            del self._engines[k0]
+        self._engine = eng
+        self._engine_key = key
+        return eng, cm
+
+    def _new_engine(self, src, opts, cm, coupling, fields, quantities):
+        """Builds a device engine for one kernel text."""
         if _native.device_count() < 1:
             raise _native.DeviceError(
                 'No HIP device found: SimulationOpenCL needs an AMD GPU.')
-        if self._engine is not None and self._dev_newer:
-            self._download_state()   # keep the state of the old engine
+        tune = self._tuning
         prog = _program(src, opts, self._device)
         cfg = _native.SimConfig()
         cfg.precision = self._precision
@@ -686,18 +766,14 @@ class SimulationOpenCL:
         cfg.nx = self._nx
         cfg.ny = self._ny
         cfg.device = self._device
-        cfg.sheet_cpt = _kernelgen.sheet_cpt(self._precision) \
-            if _kernelgen.sheet_enabled(self._precision) else 0
+        cpt = tune.get('sheet_cpt') or _kernelgen.sheet_cpt(self._precision)
+        cfg.sheet_cpt = cpt if _kernelgen.sheet_enabled(self._precision) \
+            else 0
         eng = _Engine(prog, cfg)
-        for k, v in self._options.items():
-            _native.check(_native.lib().mkh_sim_set_option(
-                eng.h, k.encode(), v), 'set_option')
+        eng.options = {}
         eng.config_version = None
         eng.graph_version = None
-        self._engine = eng
-        self._engine_key = key
-        self._host_dirty = True
-        return eng, cm
+        return eng
 
     def _configure(self, eng):
         """Sends configuration and state that changed since the engine last

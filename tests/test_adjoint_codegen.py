@@ -80,13 +80,19 @@ class Cell:
             d, ctypes.c_double, ctypes.c_double, ctypes.c_double,
             ctypes.c_double, d, d, ctypes.c_int]
         self.vjps = {}
+        defs = '#define MK_NS %d\n#define MK_NF %d\n#define MK_NFA %d\n' % (
+            self.ns, self.nf, max(1, self.nf))
         for prec, ct, rt in ((64, 'double', ctypes.c_double),
-                             (32, 'float', ctypes.c_float)):
-            src = PRELUDE % ct
-            if prec == 32:
+                             (32, 'float', ctypes.c_float),
+                             ('mixed', 'float', ctypes.c_float)):
+            src = PRELUDE % ct + defs
+            if prec != 64:
                 src += _kernelgen._FLOAT_HELPERS
-            src += generate_vjp_function(self.cm, prec) + WRAPPER
-            lib = _gcc(src, tmp_path / ('vjp%d' % prec))
+            if prec == 'mixed':
+                src += generate_vjp_function(self.cm, 32, 64) + WRAPPER
+            else:
+                src += generate_vjp_function(self.cm, prec) + WRAPPER
+            lib = _gcc(src, tmp_path / ('vjp%s' % prec))
             p = ctypes.POINTER(rt)
             lib.vjp.restype = None
             lib.vjp.argtypes = [p, rt, rt, rt, rt, p, p, p, p, p]
@@ -231,3 +237,59 @@ dot(z) = sqrt(z) + log(1 + z) * sin(z) + cos(x) * atan(z) + z^2.5 + exp(-z) / (1
         assert mu[1] == pytest.approx(exp_z, rel=1e-5 if prec == 32 else 1e-12)
         # d/dx: the taken branch 2 z has none; cos(x) atan(z) gives -sin(0)
         assert mu[0] == pytest.approx(lam[0], rel=1e-6)
+
+
+def test_mixed_precision_vjp_matches_double(tmp_path):
+    # Single-precision inputs and outputs, double-precision arithmetic: the
+    # result is the double VJP of the rounded inputs, rounded once
+    cell = Cell(tmp_path, myokit.load_model('example'), True,
+                ['ina.gNa', 'ica.gCa'])
+    rng = np.random.default_rng(8)
+    fv = np.array([16.0, 0.09])
+    for y in _states(cell):
+        # near the removable singularity of ina.m's alpha (V = -47.13)
+        for v in (y[0], -47.1299, -47.13):
+            yy = y.copy()
+            yy[0] = v
+            lam = rng.normal(size=cell.ns) * 100
+            y32 = yy.astype(np.float32).astype(float)
+            l32 = lam.astype(np.float32).astype(float)
+            f32 = fv.astype(np.float32).astype(float)
+            mm, am, fm = cell.vjp('mixed', yy, 1.0, 0.01, 0.0, 0.5, fv, lam)
+            m64, a64, f64 = cell.vjp(64, y32, np.float32(1.0),
+                                     float(np.float32(0.01)), 0.0, 0.5, f32,
+                                     l32)
+            np.testing.assert_allclose(mm, np.float32(m64), rtol=1e-6,
+                                       atol=1e-6 * np.max(np.abs(m64)))
+            assert am == pytest.approx(float(np.float32(a64)), rel=1e-6)
+            np.testing.assert_allclose(fm, np.float32(f64), rtol=1e-6,
+                                       atol=1e-6 * np.max(np.abs(f64)))
+
+
+def test_vjp_exp_against_libm(tmp_path):
+    # The double-precision exp of the mixed VJP (not the bit-exact forward
+    # one): relative error against the C library over the double range,
+    # and the limits
+    from myokit_amd._adjointgen import _VJP_EXP
+    src = '#include <math.h>\n#define __device__ static\n' \
+          '#define __forceinline__ inline\n' + _VJP_EXP + '''
+void run(const double* x, double* y, long n)
+{
+    for (long i = 0; i < n; ++i) y[i] = mk_exp_vjp(x[i]);
+}
+'''
+    lib = _gcc(src, tmp_path / 'expvjp')
+    d = ctypes.POINTER(ctypes.c_double)
+    lib.run.argtypes = [d, d, ctypes.c_long]
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-745, 709, 200000),
+                        rng.uniform(-1e-3, 1e-3, 100000),
+                        [0.0, -800.0, 800.0, np.inf, -np.inf, np.nan]])
+    y = np.zeros_like(x)
+    lib.run(x.ctypes.data_as(d), y.ctypes.data_as(d), len(x))
+    with np.errstate(over='ignore'):
+        ref = np.exp(x)
+    ok = np.isfinite(ref) & (ref >= 2.0 ** -1022)
+    assert np.max(np.abs(y[ok] - ref[ok]) / ref[ok]) < 2e-14
+    assert y[-6] == 1.0 and y[-5] == 0.0 and np.isinf(y[-4])
+    assert np.isinf(y[-3]) and y[-2] == 0.0 and np.isnan(y[-1])

@@ -190,24 +190,29 @@ t = 0 bind time
     np.testing.assert_allclose(g['c.u'], gu, rtol=1e-10)
 
 
-def test_adjoint_single_precision_against_double():
+@pytest.mark.parametrize('vjp', [64, 32])
+def test_adjoint_single_precision_against_double(vjp):
+    # Single-precision state; the VJP in double (default) or single
     dur = 3.0
     probes = [5, 20]
     out = {}
     for prec in (32, 64):
         s = cable(n=30, precision=prec)
-        adj = TissueAdjoint(s, probes, checkpoint_every=100)
+        adj = TissueAdjoint(s, probes, checkpoint_every=100,
+                            vjp_precision=vjp)
         target = np.full((adj.n_steps(dur), 2), -30.0)
         out[prec] = adj.loss_and_gradient(dur, target)
     l32, g32, x32, _ = out[32]
     l64, g64, x64, _ = out[64]
-    print('fp32 loss %.8g fp64 %.8g; dgx %.8g / %.8g' % (l32, l64, x32, x64),
-          flush=True)
+    print('vjp %d: fp32 loss %.8g fp64 %.8g; dgx %.8g / %.8g'
+          % (vjp, l32, l64, x32, x64), flush=True)
     assert rel(l32, l64) < 1e-4
-    assert rel(x32, x64) < 1e-2
+    tol = 2e-3 if vjp == 64 else 1e-2
+    assert rel(x32, x64) < tol
     for name in g64:
         err = np.max(np.abs(g32[name] - g64[name])) / np.max(np.abs(g64[name]))
-        assert err < 1e-2, name
+        print('  %s max error %.3g' % (name, err), flush=True)
+        assert err < tol, name
 
 
 def test_torch_autograd_bridge():

@@ -79,7 +79,8 @@ def test_config2_full_size_window(precision):
                   events=[(1, 0, 2, 1000, 0)], paced=(4, 1024, 0, 0))
     s = sc.product(precision)
     b = s.run_block(2.0, ['membrane.V'], log_interval=0.5)
-    assert s.engine_driver() == 'sheet'
+    # in double precision 1024 x 1024 tiles are not co-resident: grid driver
+    assert s.engine_driver() == ('sheet' if precision == 32 else 'grid')
     P = {'time': b['time'], 'V': b['membrane.V'].reshape(len(b['time']), -1)}
     r = sc.oracle().run(2.0, log_vars=['membrane.V'], log_interval=0.5)
     O = {'time': r['time'], 'V': r['membrane.V']}
@@ -231,8 +232,10 @@ def test_config5_batched_graph_adjoint_reduced():
     seeded random geometric graph (mean degree 6, g uniform in [5, 12]),
     RL, dt 0.01 ms, fields gNa, gCa, gK scaled log-uniform in [0.5, 2] per
     set. The sets run as one batch: the disjoint union of two copies of the
-    graph, one per set. Single precision with double-precision loss and
-    gradient reductions, against the same run in double precision."""
+    graph, one per set. Single-precision state with double-precision loss
+    and gradient reductions, against the same run in double precision:
+    with the default double-precision VJP arithmetic (bounded) and with a
+    single-precision VJP (error reported)."""
     from myokit_amd.adjoint import TissueAdjoint
     rng = np.random.default_rng(21)
     n = 4096
@@ -247,25 +250,35 @@ def test_config5_batched_graph_adjoint_reduced():
               for name in ('ina.gNa', 'ica.gCa', 'ik.IK.gK')}
     probes = [k * n + i for k in range(sets) for i in range(0, n, 512)]
     out = {}
-    for prec in (64, 32):
+    for prec in (64, 32, 'fp32-vjp'):
         s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
-                                    ncells=sets * n, precision=prec, rl=True)
+                                    ncells=sets * n, rl=True,
+                                    precision=64 if prec == 64 else 32)
         s.set_step_size(0.01)
         s.set_connections(allc)
         s.set_paced_cell_list(paced)
         for name, v in fields.items():
             s.set_field(name, v)
-        adj = TissueAdjoint(s, probes, checkpoint_every=500)
+        adj = TissueAdjoint(s, probes, checkpoint_every=500,
+                            vjp_precision=32 if prec == 'fp32-vjp' else None)
         target = np.full((adj.n_steps(20.0), len(probes)), -60.0)
         out[prec] = adj.loss_and_gradient(20.0, target)
     l64, g64, s64, _ = out[64]
-    l32, g32, s32, _ = out[32]
-    print('config5: loss fp32 %.8g fp64 %.8g; dL/dscale %.8g / %.8g'
-          % (l32, l64, s32, s64), flush=True)
-    assert abs(l32 - l64) <= 1e-3 * abs(l64)
-    assert abs(s32 - s64) <= 2e-2 * abs(s64)
-    for name in g64:
-        e = np.linalg.norm(g32[name] - g64[name]) / np.linalg.norm(g64[name])
-        print('config5: %s gradient relative error (fp32 vs fp64) %.3g'
-              % (name, e), flush=True)
-        assert e <= 5e-2
+    for kind, bound in ((32, 1e-3), ('fp32-vjp', None)):
+        l32, g32, s32, _ = out[kind]
+        print('config5 %s: loss %.8g vs fp64 %.8g; dL/dscale %.8g vs %.8g'
+              ' (relative error %.3g)' % (kind, l32, l64, s32, s64,
+                                          abs(s32 - s64) / abs(s64)),
+              flush=True)
+        assert abs(l32 - l64) <= 1e-3 * abs(l64)
+        errs = [abs(s32 - s64) / abs(s64)]
+        for name in g64:
+            e = np.linalg.norm(g32[name] - g64[name]) / \
+                np.linalg.norm(g64[name])
+            errs.append(e)
+            print('config5 %s: %s gradient relative error %.3g'
+                  % (kind, name, e), flush=True)
+        # The single-precision VJP is reported, not bounded: its rounding
+        # is amplified through every upstroke of the run
+        if bound is not None:
+            assert max(errs) <= bound

@@ -7,6 +7,11 @@ The voltage-clamped model: LR1991 with V demoted to a bound variable (the
 pacing level), Ca_i labelled as the membrane potential, V moved to the
 engine component. At V = -47.13 mV, ina.m's alpha is 0/0, so the cell
 clamped there by the protocol turns NaN at the first step after 1.234 ms.
+
+These simulations run in double precision. In single precision the kernel
+writes exp(k * (V + c)) as exp2(fma(V, k log2(e), c k log2(e))), which does
+not give exactly exp(0) = 1 at V = -c: the removable singularity yields a
+finite value there instead of the reference's 0/0 = NaN.
 """
 import numpy as np
 import pytest
@@ -38,7 +43,8 @@ def protocol(t=1.234):
 
 @pytest.fixture(scope='module')
 def s2():
-    s = myokit.SimulationOpenCL(clamped_model(), protocol(), ncells=(3, 3))
+    s = myokit.SimulationOpenCL(clamped_model(), protocol(), ncells=(3, 3),
+                                precision=myokit.DOUBLE_PRECISION)
     s.set_paced_cell_list([[1, 1]])
     return s
 
@@ -57,7 +63,7 @@ def test_big_steps():
 
 def test_one_over_zero(s2):
     s2.reset()
-    with pytest.raises(myokit.SimulationError, match='Time:  1.23') as e:
+    with pytest.raises(myokit.SimulationError, match='Time: +1.23') as e:
         s2.run(5)
     print(str(e.value)[:600], flush=True)
 

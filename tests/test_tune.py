@@ -1,0 +1,82 @@
+"""
+Autotuner logic without a device: candidate sets per coupling and
+geometry, tuning validation, and the choice cache.
+"""
+import pytest
+
+import myokit_amd as myokit
+from myokit_amd import tune
+
+
+def sim(ncells, **kw):
+    return myokit.SimulationOpenCL(myokit.load_model('example'),
+                                   myokit.pacing.blocktrain(1000, 2),
+                                   ncells=ncells, **kw)
+
+
+def test_candidates_by_coupling():
+    c = tune.candidates(sim((256, 256)))
+    drivers = {x['driver'] for x in c}
+    assert drivers == {'block', 'grid', 'sheet'}
+    sheet = [x for x in c if x['driver'] == 'sheet']
+    assert {x['sheet_cpt'] for x in sheet} <= set(
+        myokit_amd_choices()) and all(x['block_threads'] == 256
+                                      for x in sheet)
+    c = tune.candidates(sim(100))
+    assert {x['driver'] for x in c} == {'block', 'grid'}
+    c = tune.candidates(sim(100, diffusion=False))
+    assert {x['driver'] for x in c} == {'cells', 'grid'}
+    s = sim(10)
+    s.set_connections([(i, i + 1, 1.0) for i in range(9)])
+    assert {x['driver'] for x in tune.candidates(s)} == {'block', 'grid'}
+    for x in tune.candidates(sim((64, 64), precision=64)):
+        if x['driver'] == 'sheet':
+            assert x['sheet_cpt'] in (2, 4)
+
+
+def myokit_amd_choices():
+    from myokit_amd import _kernelgen
+    return _kernelgen.SHEET_CPT_CHOICES
+
+
+def test_set_tuning_validation():
+    s = sim(50)
+    assert s.tuning()['driver'] == 'auto'
+    with pytest.raises(ValueError):
+        s.set_tuning(block_threads=100)
+    with pytest.raises(ValueError):
+        s.set_tuning(sheet_cpt=3)
+    with pytest.raises(ValueError):
+        s.set_tuning(sheet_waves=0)
+    with pytest.raises(ValueError):
+        s.set_tuning(driver='fast')
+    s.set_tuning(driver='grid', block_threads=128, sheet_cpt=8,
+                 sheet_waves=3)
+    t = s.tuning()
+    assert t == {'driver': 'grid', 'block_threads': 128, 'sheet_cpt': 8,
+                 'sheet_waves': 3}
+
+
+def test_cache_round_trip(tmp_path, monkeypatch):
+    monkeypatch.setenv(tune.CACHE_ENV, str(tmp_path / 'tune.json'))
+    s = sim((32, 32))
+    assert tune.apply_cached(s) is None
+    cfg = {'driver': 'grid', 'block_threads': 512}
+    tune._store(tune._key(s), cfg)
+    t = sim((32, 32))
+    assert tune.apply_cached(t) == cfg
+    assert t.tuning()['block_threads'] == 512
+    # another geometry or precision has its own entry
+    assert tune.apply_cached(sim((32, 16))) is None
+    assert tune.apply_cached(sim((32, 32), precision=64)) is None
+
+
+def test_tuned_kernel_source_has_tile_shape():
+    from myokit_amd import _kernelgen
+    from myokit_amd.simulation import _prepare_cell_model
+    cm = _prepare_cell_model(myokit.load_model('example'), rl=True,
+                             diffusion=True, fields=[], quantities=[])
+    src = _kernelgen.generate_kernel(cm, 32, 1, tile_cpt=8, tile_waves=3)
+    assert '#define MK_SCPT 8' in src and '#define MK_SWAVES 3' in src
+    with pytest.raises(ValueError):
+        _kernelgen.generate_kernel(cm, 32, 1, tile_cpt=16)
