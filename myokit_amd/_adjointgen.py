@@ -31,6 +31,9 @@ class _Rev:
         self.cm = cm
         self.p = precision
         self.w = cell_writer(cm, precision, False)
+        if precision == DOUBLE_PRECISION:
+            # the VJP's forward sweep needs no exactly rounded division
+            self.w.rcp64 = 'mk_rcp64'
         self.bars = {}          # LhsExpression -> C name of its adjoint
         states = set(cm.states)
         for s in cm.states:
@@ -47,6 +50,8 @@ class _Rev:
             if what == 'idiff':
                 self.bars[ex.Name(var)] = 'b_idiff'
         self.out = []
+        self._ntmp = 0
+        self._inf = {}          # Variable -> may its value be infinite
 
     def lit(self, v):
         return real_literal(v, self.p)
@@ -55,11 +60,52 @@ class _Rev:
         """C code of the forward value of sub-expression ``e``."""
         return self.w.ex(e)
 
-    def mul(self, g, v):
-        """Adjoint ``g`` times forward value ``v``, zero when ``g`` is zero
-        (so an overflowed forward value met by a zero adjoint gives 0, not
+    def tmp(self, code):
+        """Binds ``code`` to a new local and returns its name."""
+        name = 't' + str(self._ntmp)
+        self._ntmp += 1
+        self.out.append('    const Real ' + name + ' = ' + code + ';')
+        return name
+
+    def may_inf(self, e):
+        """False if the forward value of ``e`` is finite whenever the
+        states, fields and inputs are: no exp, division, power, logarithm,
+        tangent or remainder below it."""
+        if isinstance(e, ex.Number):
+            x = e.eval()
+            return x != x or abs(x) == float('inf')
+        if isinstance(e, ex.Derivative):
+            return True
+        if isinstance(e, ex.Name):
+            var = e.var()
+            if not hasattr(var, 'is_state') or var.is_state() or \
+                    var in self.cm.fields or var in self.cm.bound:
+                return False
+            r = self._inf.get(var)
+            if r is None:
+                self._inf[var] = True       # provisional, for safety
+                r = self._inf[var] = self.may_inf(var.rhs())
+            return r
+        if isinstance(e, (ex.Exp, ex.Divide, ex.Power, ex.Log, ex.Log10,
+                          ex.Tan, ex.Quotient, ex.Remainder)):
+            return True
+        return any(self.may_inf(x) for x in e)
+
+    def mul(self, g, v, e=None):
+        """Adjoint ``g`` times forward value ``v`` (the value of ``e``). If
+        that value may be infinite, the product is zero when ``g`` is zero
+        (an overflowed forward value met by a zero adjoint gives 0, not
         0 * inf = NaN)."""
+        if e is not None and not self.may_inf(e):
+            return '(' + g + ' * ' + v + ')'
         return 'mk_gmul(' + g + ', ' + v + ')'
+
+    def recip(self, b):
+        """1 / b: single precision by the hardware reciprocal, double
+        precision by the reciprocal with two Newton steps (mk_rcp64)."""
+        if self.p == DOUBLE_PRECISION:
+            return 'mk_rcp64(' + b + ')'
+        return '__builtin_amdgcn_rcpf(' + b + ')'
 
     def div(self, a, b):
         if self.p == DOUBLE_PRECISION:
@@ -99,15 +145,17 @@ class _Rev:
             return self.rev(e[1], '(-' + g + ')', guard)
         if isinstance(e, ex.Multiply):
             a, b = e
-            self.rev(a, self.mul(g, self.v(b)), guard)
-            return self.rev(b, self.mul(g, self.v(a)), guard)
+            self.rev(a, self.mul(g, self.v(b), b), guard)
+            return self.rev(b, self.mul(g, self.v(a), a), guard)
         if isinstance(e, ex.Divide):
             a, b = e
-            vb = self.v(b)
-            self.rev(a, self.gdiv(g, vb), guard)
-            # d(a/b)/db = -(a/b)/b
-            q = self.div(self.v(e), vb)
-            return self.rev(b, self.mul('(-' + g + ')', q), guard)
+            # one reciprocal for both operands: d/da = 1/b,
+            # d/db = -(a/b)/b
+            rb = self.tmp(self.recip(self.v(b)))
+            gt = self.tmp(g)
+            self.rev(a, self.mul(gt, rb), guard)
+            return self.rev(b, self.mul('(-' + gt + ')', '(' + self.v(e)
+                                        + ' * ' + rb + ')'), guard)
         if isinstance(e, (ex.Quotient, ex.Floor, ex.Ceil)):
             return
         if isinstance(e, ex.Remainder):
@@ -201,8 +249,9 @@ _VJP_EXP = """
  * and 2^k by ldexp, which gives inf and 0 beyond the range. */
 __device__ __forceinline__ double mk_exp_vjp(double x)
 {
-    /* beyond these bounds the result is 0 or inf anyway; NaN passes */
-    x = x < -760.0 ? -760.0 : (x > 720.0 ? 720.0 : x);
+    /* beyond these bounds the result is 0 or inf anyway (a NaN argument
+     * gives 0: it can only come from a forward state that is NaN) */
+    x = __builtin_fmin(__builtin_fmax(x, -760.0), 720.0);
     const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
     const double r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56,
                                    __builtin_fma(-k, 0x1.62e42fefa39efp-1,
@@ -254,6 +303,20 @@ __device__ __forceinline__ void mk_cell_vjp(
 """
 
 
+_VJP_RCP64 = """
+/* 1/b in double precision for the VJP: the hardware reciprocal refined by
+ * two Newton steps, r <- r + r (1 - b r); for b = 0, inf or NaN the
+ * refinement would give NaN, so the plain reciprocal is returned. */
+__device__ __forceinline__ double mk_rcp64(double b)
+{
+    const double r0 = __builtin_amdgcn_rcp(b);
+    double r = __builtin_fma(__builtin_fma(-b, r0, 1.0), r0, r0);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    return __builtin_isfinite(r) ? r : r0;
+}
+"""
+
+
 def generate_vjp_function(cm, precision, compute=None):
     """HIP source of ``mk_cell_vjp`` for a CellModel. ``compute`` is the
     precision of the VJP's arithmetic (default: ``precision``); double
@@ -261,7 +324,11 @@ def generate_vjp_function(cm, precision, compute=None):
     ``mk_cell_vjp_f64`` and a wrapper that converts on entry and exit."""
     import re
     if compute is None or compute == precision:
-        return _vjp_source(cm, precision)
+        src = _vjp_source(cm, precision)
+        if precision == DOUBLE_PRECISION:
+            src = _VJP_EXP + _VJP_RCP64 + re.sub(r'\bmk_exp\(', 'mk_exp_vjp(',
+                                                src)
+        return src
     if not (precision != DOUBLE_PRECISION and compute == DOUBLE_PRECISION):
         raise ValueError('Only a double-precision VJP for a single-precision'
                          ' kernel can be mixed.')
@@ -270,7 +337,7 @@ def generate_vjp_function(cm, precision, compute=None):
     for name in ('mk_cell_vjp', 'mk_gmul', 'mk_gdiv'):
         src = re.sub(r'\b' + name + r'\b', name + '_f64', src)
     src = re.sub(r'\bmk_exp\(', 'mk_exp_vjp(', src)
-    return _VJP_EXP + src + _MIXED_WRAPPER
+    return _VJP_EXP + _VJP_RCP64 + src + _MIXED_WRAPPER
 
 
 def _vjp_source(cm, precision):

@@ -139,9 +139,21 @@ class HipExpressionWriter:
     def _ex_Multiply(self, e):
         return self._infix(e, '*')
 
+    # Double precision divides with IEEE '/' (bit-exact with the oracle).
+    # Set to a function name (e.g. 'mk_rcp64') to write a * f(b) instead,
+    # where exact rounding is not required (the adjoint's forward sweep).
+    rcp64 = None
+
     def _ex_Divide(self, e):
         if self.precision == DOUBLE_PRECISION:
-            return self._infix(e, '/')
+            if self.rcp64 is None:
+                return self._infix(e, '/')
+            a, b = e
+            if isinstance(b, ex.Number):
+                return ('(' + self.ex(a) + ' * '
+                        + real_literal(1.0 / b.eval(), self.precision) + ')')
+            return ('(' + self.ex(a) + ' * ' + self.rcp64 + '(' + self.ex(b)
+                    + '))')
         # Single precision: a * rcp(b) (v_rcp_f32, within the 2.5 ulp that
         # OpenCL allows for single-precision division); a literal divisor
         # becomes a multiplication by its reciprocal.
