@@ -24,6 +24,76 @@ from ._kernelgen import (DOUBLE_PRECISION, _cname, cell_writer,
                          forward_sweep, real_literal)
 
 
+def _nonneg(e, cm, seen=None):
+    """True if the value of ``e`` can be shown to be >= 0 (or NaN) from its
+    form alone: non-negative numbers, exp, sqrt, and sums, products and
+    quotients of those, following intermediate variables."""
+    if isinstance(e, ex.Number):
+        return e.eval() >= 0
+    if isinstance(e, (ex.Exp, ex.Sqrt)):
+        return True
+    if isinstance(e, (ex.Plus, ex.Multiply, ex.Divide)):
+        return _nonneg(e[0], cm, seen) and _nonneg(e[1], cm, seen)
+    if isinstance(e, ex.Power) and isinstance(e[1], ex.Number):
+        x = e[1].eval()
+        return x == int(x) and int(x) % 2 == 0 or _nonneg(e[0], cm, seen)
+    if isinstance(e, ex.Name):
+        var = e.var()
+        if var in cm.constants:
+            return cm.constants[var] >= 0
+        seen = set() if seen is None else seen
+        if var in cm.order and not var.is_state() and var not in seen:
+            seen.add(var)
+            return _nonneg(var.rhs(), cm, seen)
+    return False
+
+
+def _may_cancel(e, cm, seen=None):
+    """True if a multiplicative factor of ``e`` is a sum or difference
+    that can cancel to zero (following intermediate variables)."""
+    if seen is None:
+        seen = set()
+    if isinstance(e, ex.Minus):
+        return True
+    if isinstance(e, ex.Plus):
+        return not (_nonneg(e[0], cm) and _nonneg(e[1], cm))
+    if isinstance(e, ex.Multiply):
+        return _may_cancel(e[0], cm, seen) or _may_cancel(e[1], cm, seen)
+    if isinstance(e, (ex.Divide, ex.PrefixMinus, ex.PrefixPlus)):
+        return _may_cancel(e[0], cm, seen)
+    if isinstance(e, ex.Power) and isinstance(e[1], ex.Number) \
+            and e[1].eval() > 0:
+        return _may_cancel(e[0], cm, seen)
+    if isinstance(e, ex.Name):
+        var = e.var()
+        if var in cm.order and not var.is_state() and var not in seen:
+            seen.add(var)
+            return _may_cancel(var.rhs(), cm, seen)
+    return False
+
+
+def _variable(e, cm):
+    """True if ``e`` depends on anything but literal constants."""
+    if isinstance(e, ex.Number):
+        return False
+    if isinstance(e, ex.Name):
+        return e.var() not in cm.constants
+    return any(_variable(x, cm) for x in e)
+
+
+def needs_island(e, cm):
+    """True if ``e`` holds a quotient N / D with a denominator factor that
+    can cancel and a numerator that is not constant: the form of a
+    removable singularity, such as 0.32 (V + 47.13) / (1 - exp(-0.1 (V +
+    47.13))) or (exp(0.04 (V + 77)) - 1) / ((V + 77) exp(...)). Near it the
+    quotient rule subtracts two large, nearly equal terms, and a
+    single-precision VJP keeps few correct digits of the derivative."""
+    if isinstance(e, ex.Divide) and _may_cancel(e[1], cm) \
+            and _variable(e[0], cm):
+        return True
+    return any(needs_island(x, cm) for x in e)
+
+
 class _Rev:
     """Reverse-mode code emitter for one cell."""
 
@@ -317,21 +387,111 @@ __device__ __forceinline__ double mk_rcp64(double b)
 """
 
 
-def generate_vjp_function(cm, precision, compute=None):
+_VJP_ONEM_EXPF = """
+/* 1 - exp(-a) in single precision, for the Rush-Larsen partial
+ * d y' / d inf. Formed as 1 - E it loses log2(1/a) bits to cancellation when
+ * dt << tau (a = dt / tau near 0), and the adjoint amplifies that error
+ * through every upstroke. Below |a| = 0.5 the Taylor series of
+ * (1 - exp(-a)) / a to degree 7 (truncation under 1e-8 relative) is used,
+ * above it 1 - E with E = exp(-a), which is then exact to a few ulp. */
+__device__ __forceinline__ float mk_onem_expf(const float a, const float E)
+{
+    float p = -0x1.a01a02p-16f;
+    p = __builtin_fmaf(p, a, 0x1.a01a02p-13f);
+    p = __builtin_fmaf(p, a, -0x1.6c16c2p-10f);
+    p = __builtin_fmaf(p, a, 0x1.111112p-7f);
+    p = __builtin_fmaf(p, a, -0x1.555556p-5f);
+    p = __builtin_fmaf(p, a, 0x1.555556p-3f);
+    p = __builtin_fmaf(p, a, -0.5f);
+    p = __builtin_fmaf(p, a, 1.0f);
+    return __builtin_fabsf(a) < 0.5f ? a * p : 1.0f - E;
+}
+"""
+
+
+def generate_vjp_function(cm, precision, compute=None, islands=True):
     """HIP source of ``mk_cell_vjp`` for a CellModel. ``compute`` is the
     precision of the VJP's arithmetic (default: ``precision``); double
     arithmetic for a single-precision kernel gives a double-precision
-    ``mk_cell_vjp_f64`` and a wrapper that converts on entry and exit."""
+    ``mk_cell_vjp_f64`` and a wrapper that converts on entry and exit.
+
+    A single-precision VJP evaluates the reverse rule of each equation that
+    holds a removable singularity (``needs_island``) in double precision,
+    and rounds only the sums it adds to other adjoints (unless ``islands``
+    is false)."""
     import re
     if compute is None or compute == precision:
-        src = _vjp_source(cm, precision)
+        src = _vjp_source(cm, precision, islands)
         if precision == DOUBLE_PRECISION:
-            src = _VJP_EXP + _VJP_RCP64 + re.sub(r'\bmk_exp\(', 'mk_exp_vjp(',
-                                                src)
-        return src
+            return _VJP_EXP + _VJP_RCP64 + re.sub(r'\bmk_exp\(',
+                                                'mk_exp_vjp(', src)
+        head = _VJP_LMUL + _VJP_ONEM_EXPF
+        if 'mk_rcp64' in src or 'mk_exp_vjp' in src or '_f64(' in src:
+            head = _VJP_EXP + _VJP_RCP64 + _ISLAND_GUARDS + head
+        return head + src
     if not (precision != DOUBLE_PRECISION and compute == DOUBLE_PRECISION):
         raise ValueError('Only a double-precision VJP for a single-precision'
                          ' kernel can be mixed.')
+    return _mixed_source(cm)
+
+
+#: Single-precision multiply with 0 * x = 0 for every x, inf and NaN
+#: included (v_mul_legacy_f32): the adjoint guard in one instruction
+_VJP_LMUL = """
+__device__ __forceinline__ float mk_lmul(const float g, const float v)
+{
+    float r;
+    asm("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(g), "v"(v));
+    return r;
+}
+"""
+
+
+_ISLAND_GUARDS = """
+__device__ __forceinline__ double mk_gmul_f64(const double g, const double v)
+{
+    return g == 0.0 ? 0.0 : g * v;
+}
+
+__device__ __forceinline__ double mk_gdiv_f64(const double g, const double b)
+{
+    return g == 0.0 ? 0.0 : g / b;
+}
+"""
+
+
+def _island(r, e, target, tab):
+    """Reverse rule of ``e`` (adjoint in ``target``) in double precision,
+    inside a single-precision VJP: forward values are read as doubles,
+    leaf contributions are summed in double and rounded once."""
+    import re
+    r64 = _Rev(r.cm, DOUBLE_PRECISION)
+    r64.w.names = {k: '((double)' + v + ')' for k, v in r64.w.names.items()}
+    r64._ntmp = r._ntmp
+    r64._inf = r._inf
+    for k in list(r64.bars):
+        r64.bars[k] = 'isl_' + r.bars[k]
+    r64.rev(e, 'isl_g')
+    r._ntmp = r64._ntmp
+    body = '\n'.join(r64.out)
+    body = re.sub(r'\bReal\b', 'double', body)
+    body = re.sub(r'\bmk_gmul\(', 'mk_gmul_f64(', body)
+    body = re.sub(r'\bmk_gdiv\(', 'mk_gdiv_f64(', body)
+    body = re.sub(r'\bmk_exp\(', 'mk_exp_vjp(', body)
+    used = sorted(set(re.findall(r'\bisl_\w+', body)) - {'isl_g'})
+    lines = [tab + '{', tab + tab + 'const double isl_g = (double)' + target
+             + ';']
+    lines += [tab + tab + 'double ' + u + ' = 0.0;' for u in used]
+    lines += [tab + line for line in body.split('\n') if line]
+    lines += [tab + tab + u[4:] + ' += (Real)' + u + ';' for u in used]
+    lines.append(tab + '}')
+    return lines
+
+
+def _mixed_source(cm):
+    """Double-precision ``mk_cell_vjp_f64`` and the single-precision
+    ``mk_cell_vjp`` wrapper around it."""
+    import re
     src = _vjp_source(cm, DOUBLE_PRECISION)
     src = re.sub(r'\bReal\b', 'double', src)
     for name in ('mk_cell_vjp', 'mk_gmul', 'mk_gdiv'):
@@ -340,20 +500,29 @@ def generate_vjp_function(cm, precision, compute=None):
     return _VJP_EXP + _VJP_RCP64 + src + _MIXED_WRAPPER
 
 
-def _vjp_source(cm, precision):
-    """HIP source of ``mk_cell_vjp`` computing in ``precision``."""
+def _vjp_source(cm, precision, islands=True):
+    """HIP source of ``mk_cell_vjp`` computing in ``precision``, with
+    double-precision islands (``_island``) in single precision."""
     r = _Rev(cm, precision)
     w = r.w
     tab = '    '
+    if precision == DOUBLE_PRECISION:
+        gmul = 'g == (Real)0 ? (Real)0 : g * v'
+        gdiv = 'g == (Real)0 ? (Real)0 : ' + r.div('g', 'b')
+    else:
+        # The legacy multiply gives 0 when either factor is 0, whatever the
+        # other (inf and NaN included): the guard in one instruction
+        gmul = 'mk_lmul(g, v)'
+        gdiv = 'mk_lmul(g, __builtin_amdgcn_rcpf(b))'
     lines = [
         '__device__ __forceinline__ Real mk_gmul(const Real g, const Real v)',
         '{',
-        tab + 'return g == (Real)0 ? (Real)0 : g * v;',
+        tab + 'return ' + gmul + ';',
         '}',
         '',
         '__device__ __forceinline__ Real mk_gdiv(const Real g, const Real b)',
         '{',
-        tab + 'return g == (Real)0 ? (Real)0 : ' + r.div('g', 'b') + ';',
+        tab + 'return ' + gdiv + ';',
         '}',
         '',
         '__device__ __forceinline__ void mk_cell_vjp(',
@@ -368,28 +537,44 @@ def _vjp_source(cm, precision):
     lines.extend(forward_sweep(cm, w, tab))
     for b in sorted(set(r.bars.values())):
         lines.append(tab + 'Real ' + b + ' = (Real)0;')
-    # Outputs: the state update of every state
+    # Outputs: the state update of every state. The direct terms of
+    # mu = dy'/dy^T lam (lam for forward Euler, lam E for Rush-Larsen) are
+    # kept out of the accumulators: the reverse sweep adds dozens of
+    # contributions of order dt to each state's adjoint, and summed into an
+    # accumulator that already holds lam, each would be rounded to lam's
+    # precision. In single precision that is several ulp of lam per step,
+    # the same at every step at rest, and so a drift of the adjoint.
     expf = w.fn('exp')
+    direct = []
     for i, s in enumerate(cm.states):
         x = _cname(s, 'x_')
         if s in cm.rl_states:
             inf, tau = cm.rl_states[s]
             vi, vt = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
             E = 'E_' + str(i)
-            lines.append(tab + 'const Real ' + E + ' = ' + expf + '('
-                         + r.div('-dt', vt) + ');')
+            if precision == DOUBLE_PRECISION:
+                lines.append(tab + 'const Real ' + E + ' = ' + expf + '('
+                             + r.div('-dt', vt) + ');')
+                D = '((Real)1 - ' + E + ')'
+            else:
+                A, D = 'A_' + str(i), 'D_' + str(i)
+                lines.append(tab + 'const Real ' + A + ' = '
+                             + r.div('dt', vt) + ';')
+                lines.append(tab + 'const Real ' + E + ' = ' + expf + '(-'
+                             + A + ');')
+                lines.append(tab + 'const Real ' + D + ' = mk_onem_expf('
+                             + A + ', ' + E + ');')
             # y' = inf - (inf - x) E: dy'/dx = E, dy'/dinf = 1 - E,
-            # dy'/dtau = -(inf - x) E dt / tau^2
-            lines.append(tab + _cname(s, 'bx_') + ' += lam[' + str(i)
-                         + '] * ' + E + ';')
+            # dy'/dtau = -(inf - x) E dt / tau^2. The direct term lam E is
+            # added to mu at the end (see below).
+            direct.append('lam[' + str(i) + '] * ' + E)
             lines.append(tab + r.bars[ex.Name(inf)] + ' += lam[' + str(i)
-                         + '] * ((Real)1 - ' + E + ');')
+                         + '] * ' + D + ';')
             lines.append(tab + r.bars[ex.Name(tau)] + ' += -lam[' + str(i)
                          + '] * (' + vi + ' - ' + x + ') * ' + E + ' * '
                          + r.div('dt', '(' + vt + ' * ' + vt + ')') + ';')
         else:
-            lines.append(tab + _cname(s, 'bx_') + ' += lam[' + str(i)
-                         + '];')
+            direct.append('lam[' + str(i) + ']')
             lines.append(tab + _cname(s, 'bd_') + ' += dt * lam[' + str(i)
                          + '];')
     # Reverse dependency order
@@ -397,14 +582,20 @@ def _vjp_source(cm, precision):
         if var.is_state():
             if var in cm.rl_states:
                 continue
-            r.rev(var.rhs(), r.bars[ex.Derivative(ex.Name(var))])
+            target = r.bars[ex.Derivative(ex.Name(var))]
         else:
-            r.rev(var.rhs(), r.bars[ex.Name(var)])
+            target = r.bars[ex.Name(var)]
+        if islands and precision != DOUBLE_PRECISION and \
+                needs_island(var.rhs(), cm):
+            r.out.extend(_island(r, var.rhs(), target, tab))
+        else:
+            r.rev(var.rhs(), target)
     for var in reversed(cm.per_cell_constants):
         r.rev(var.rhs(), r.bars[ex.Name(var)])
     lines.extend(r.out)
     for i, s in enumerate(cm.states):
-        lines.append(tab + 'mu[' + str(i) + '] = ' + _cname(s, 'bx_') + ';')
+        lines.append(tab + 'mu[' + str(i) + '] = ' + direct[i] + ' + '
+                     + _cname(s, 'bx_') + ';')
     lines.append(tab + '*abar = ' + ('b_idiff' if 'b_idiff' in
                                      r.bars.values() else '(Real)0') + ';')
     for k in range(len(cm.fields)):

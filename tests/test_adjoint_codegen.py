@@ -20,7 +20,7 @@ import pytest
 
 import myokit_amd as myokit
 from myokit_amd import _kernelgen
-from myokit_amd._adjointgen import generate_vjp_function
+from myokit_amd._adjointgen import _VJP_LMUL, generate_vjp_function
 from myokit_amd.simulation import _prepare_model
 
 from oracle import cgen
@@ -39,6 +39,13 @@ PRELUDE = '''
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 #define __builtin_amdgcn_rcp(x) (1.0 / (x))
 typedef %s Real;
+'''
+
+LMUL_HOST = '''
+static inline float mk_lmul(float g, float v)
+{
+    return (g == 0.0f || v == 0.0f) ? 0.0f : g * v;
+}
 '''
 
 WRAPPER = '''
@@ -85,14 +92,20 @@ class Cell:
             self.ns, self.nf, max(1, self.nf))
         for prec, ct, rt in ((64, 'double', ctypes.c_double),
                              (32, 'float', ctypes.c_float),
+                             ('plain', 'float', ctypes.c_float),
                              ('mixed', 'float', ctypes.c_float)):
             src = PRELUDE % ct + defs
             if prec != 64:
                 src += _kernelgen._FLOAT_HELPERS
             if prec == 'mixed':
                 src += generate_vjp_function(self.cm, 32, 64) + WRAPPER
+            elif prec == 'plain':
+                src += generate_vjp_function(self.cm, 32, islands=False) \
+                    + WRAPPER
             else:
                 src += generate_vjp_function(self.cm, prec) + WRAPPER
+            # host stand-in for the legacy multiply (0 * x = 0 for every x)
+            src = src.replace(_VJP_LMUL, LMUL_HOST)
             lib = _gcc(src, tmp_path / ('vjp%s' % prec))
             p = ctypes.POINTER(rt)
             lib.vjp.restype = None
@@ -294,3 +307,55 @@ void run(const double* x, double* y, long n)
     assert np.max(np.abs(y[ok] - ref[ok]) / ref[ok]) < 2e-14
     assert y[-6] == 1.0 and y[-5] == 0.0 and np.isinf(y[-4])
     assert np.isinf(y[-3]) and y[-2] == 0.0
+
+
+def test_single_precision_rl_partials_at_rest(tmp_path):
+    # At rest the slow gates have dt / tau ~ 1e-4: d y' / dV contains
+    # inf'(V) (1 - E) with E = exp(-dt / tau) ~ 1, which formed as 1 - E in
+    # single precision keeps only a few correct digits. The VJP forms it as
+    # -expm1(-dt / tau): every d gate' / dV agrees with the double VJP of
+    # the same inputs to 1e-5 relative.
+    cell = Cell(tmp_path, myokit.load_model('example'), True,
+                ['ina.gNa', 'ica.gCa'])
+    fv = np.array([16.0, 0.09])
+    y = cell.y0.astype(np.float32).astype(float)
+    iv = [s.qname() for s in cell.cm.states].index('membrane.V')
+    for k, s in enumerate(cell.cm.states):
+        if s not in cell.cm.rl_states:
+            continue
+        lam = np.zeros(cell.ns)
+        lam[k] = 1.0
+        m32 = cell.vjp(32, y, 1.0, 0.01, 0.0, 0.0, fv, lam)[0]
+        m64 = cell.vjp(64, y, float(np.float32(1.0)),
+                       float(np.float32(0.01)), 0.0, 0.0,
+                       fv.astype(np.float32).astype(float), lam)[0]
+        assert m32[iv] == pytest.approx(m64[iv], rel=1e-5), s.qname()
+
+
+def test_onem_expf_against_libm(tmp_path):
+    # 1 - exp(-a) of the single-precision VJP against double expm1, over
+    # the series range, across its switch point and beyond
+    from myokit_amd._adjointgen import _VJP_ONEM_EXPF
+    src = '#include <math.h>\n#define __device__ static\n' \
+          '#define __forceinline__ inline\n' \
+          '#define __builtin_fmaf fmaf\n#define __builtin_fabsf fabsf\n' \
+          + _VJP_ONEM_EXPF + '''
+void run(const float* a, float* y, long n)
+{
+    for (long i = 0; i < n; ++i) y[i] = mk_onem_expf(a[i], expf(-a[i]));
+}
+'''
+    lib = _gcc(src, tmp_path / 'onem')
+    f = ctypes.POINTER(ctypes.c_float)
+    lib.run.argtypes = [f, f, ctypes.c_long]
+    rng = np.random.default_rng(2)
+    a = np.concatenate([10.0 ** rng.uniform(-9, 1.5, 100000),
+                        -(10.0 ** rng.uniform(-9, 1, 20000)),
+                        [0.0, 0.49999997, 0.5, 0.50000006, np.inf]])
+    a = a.astype(np.float32)
+    y = np.zeros_like(a)
+    lib.run(a.ctypes.data_as(f), y.ctypes.data_as(f), len(a))
+    ref = -np.expm1(-a.astype(float))
+    ok = np.isfinite(ref) & (ref != 0)
+    assert np.max(np.abs(y[ok] - ref[ok]) / np.abs(ref[ok])) < 4e-7
+    assert y[-5] == 0.0 and y[-1] == 1.0

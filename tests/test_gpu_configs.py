@@ -250,7 +250,7 @@ def test_config5_batched_graph_adjoint_reduced():
               for name in ('ina.gNa', 'ica.gCa', 'ik.IK.gK')}
     probes = [k * n + i for k in range(sets) for i in range(0, n, 512)]
     out = {}
-    for prec in (64, 32, 'fp32-vjp'):
+    for prec in (64, 32, 'f64-vjp'):
         s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
                                     ncells=sets * n, rl=True,
                                     precision=64 if prec == 64 else 32)
@@ -260,11 +260,11 @@ def test_config5_batched_graph_adjoint_reduced():
         for name, v in fields.items():
             s.set_field(name, v)
         adj = TissueAdjoint(s, probes, checkpoint_every=500,
-                            vjp_precision=32 if prec == 'fp32-vjp' else None)
+                            vjp_precision=64 if prec == 'f64-vjp' else None)
         target = np.full((adj.n_steps(20.0), len(probes)), -60.0)
         out[prec] = adj.loss_and_gradient(20.0, target)
     l64, g64, s64, _ = out[64]
-    for kind, bound in ((32, 1e-3), ('fp32-vjp', None)):
+    for kind, bound in ((32, 1e-3), ('f64-vjp', 1e-3)):
         l32, g32, s32, _ = out[kind]
         print('config5 %s: loss %.8g vs fp64 %.8g; dL/dscale %.8g vs %.8g'
               ' (relative error %.3g)' % (kind, l32, l64, s32, s64,
@@ -278,7 +278,4 @@ def test_config5_batched_graph_adjoint_reduced():
             errs.append(e)
             print('config5 %s: %s gradient relative error %.3g'
                   % (kind, name, e), flush=True)
-        # The single-precision VJP is reported, not bounded: its rounding
-        # is amplified through every upstroke of the run
-        if bound is not None:
-            assert max(errs) <= bound
+        assert max(errs) <= bound
