@@ -426,8 +426,8 @@ def generate_vjp_function(cm, precision, compute=None, islands=True):
             return _VJP_EXP + _VJP_RCP64 + re.sub(r'\bmk_exp\(',
                                                 'mk_exp_vjp(', src)
         head = _VJP_LMUL + _VJP_ONEM_EXPF
-        if 'mk_rcp64' in src or 'mk_exp_vjp' in src or '_f64(' in src:
-            head = _VJP_EXP + _VJP_RCP64 + _ISLAND_GUARDS + head
+        if 'mk_rcp64' in src or 'mk_exp_isl' in src or '_f64(' in src:
+            head = _ISLAND_EXP + _VJP_RCP64 + _ISLAND_GUARDS + head
         return head + src
     if not (precision != DOUBLE_PRECISION and compute == DOUBLE_PRECISION):
         raise ValueError('Only a double-precision VJP for a single-precision'
@@ -447,6 +447,31 @@ __device__ __forceinline__ float mk_lmul(const float g, const float v)
 """
 
 
+_ISLAND_EXP = """
+/* exp in double precision for the double-precision islands of a
+ * single-precision VJP, where 1e-11 relative is ample: x = k ln2 + r,
+ * exp(r) = exp(r / 16)^16 with exp(r / 16) by its Taylor series to degree
+ * 5 (truncation 1.5e-13, 2.3e-12 after the four squarings), 2^k by
+ * ldexp. Three constants instead of eleven. */
+__device__ __forceinline__ double mk_exp_isl(double x)
+{
+    x = __builtin_fmin(__builtin_fmax(x, -760.0), 720.0);
+    const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+    const double s = __builtin_fma(-k, 0x1.62e42fefa39efp-5, x * 0.0625);
+    double p = __builtin_fma(s, 0x1.1111111111111p-7, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, s, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, s, 0.5);
+    p = __builtin_fma(p, s, 1.0);
+    p = __builtin_fma(p, s, 1.0);
+    p = p * p;
+    p = p * p;
+    p = p * p;
+    p = p * p;
+    return __builtin_ldexp(p, (int)k);
+}
+"""
+
+
 _ISLAND_GUARDS = """
 __device__ __forceinline__ double mk_gmul_f64(const double g, const double v)
 {
@@ -460,32 +485,37 @@ __device__ __forceinline__ double mk_gdiv_f64(const double g, const double b)
 """
 
 
-def _island(r, e, target, tab):
-    """Reverse rule of ``e`` (adjoint in ``target``) in double precision,
-    inside a single-precision VJP: forward values are read as doubles,
-    leaf contributions are summed in double and rounded once."""
+def _island(r, e, k, tab):
+    """Partial derivatives of ``e`` with respect to its leaves, computed in
+    double precision right after the forward sweep (the reverse rule of
+    ``e`` seeded with 1), where few single-precision values are live, and
+    rounded once. Returns the lines that compute them and a map from each
+    leaf's adjoint to the name of its partial derivative; the reverse
+    sweep then adds ``adjoint(e) * partial`` in single precision."""
     import re
     r64 = _Rev(r.cm, DOUBLE_PRECISION)
-    r64.w.names = {k: '((double)' + v + ')' for k, v in r64.w.names.items()}
+    r64.w.names = {key: '((double)' + v + ')'
+                   for key, v in r64.w.names.items()}
     r64._ntmp = r._ntmp
     r64._inf = r._inf
-    for k in list(r64.bars):
-        r64.bars[k] = 'isl_' + r.bars[k]
-    r64.rev(e, 'isl_g')
+    for key in list(r64.bars):
+        r64.bars[key] = 'isl_' + r.bars[key]
+    r64.rev(e, '1.0')
     r._ntmp = r64._ntmp
     body = '\n'.join(r64.out)
     body = re.sub(r'\bReal\b', 'double', body)
     body = re.sub(r'\bmk_gmul\(', 'mk_gmul_f64(', body)
     body = re.sub(r'\bmk_gdiv\(', 'mk_gdiv_f64(', body)
-    body = re.sub(r'\bmk_exp\(', 'mk_exp_vjp(', body)
-    used = sorted(set(re.findall(r'\bisl_\w+', body)) - {'isl_g'})
-    lines = [tab + '{', tab + tab + 'const double isl_g = (double)' + target
-             + ';']
+    body = re.sub(r'\bmk_exp\(', 'mk_exp_isl(', body)
+    used = sorted(set(re.findall(r'\bisl_\w+', body)))
+    jac = {u[4:]: 'J' + str(k) + '_' + u[4:] for u in used}
+    lines = [tab + 'Real ' + jac[u[4:]] + ';' for u in used]
+    lines.append(tab + '{')
     lines += [tab + tab + 'double ' + u + ' = 0.0;' for u in used]
     lines += [tab + line for line in body.split('\n') if line]
-    lines += [tab + tab + u[4:] + ' += (Real)' + u + ';' for u in used]
+    lines += [tab + tab + jac[u[4:]] + ' = (Real)' + u + ';' for u in used]
     lines.append(tab + '}')
-    return lines
+    return lines, jac
 
 
 def _mixed_source(cm):
@@ -535,6 +565,14 @@ def _vjp_source(cm, precision, islands=True):
         tab + '(void)t; (void)dt; (void)pace; (void)idiff; (void)fv;',
     ]
     lines.extend(forward_sweep(cm, w, tab))
+    islands_jac = {}
+    if islands and precision != DOUBLE_PRECISION:
+        for var in cm.order:
+            if not (var.is_state() and var in cm.rl_states) and \
+                    needs_island(var.rhs(), cm):
+                pre, jac = _island(r, var.rhs(), len(islands_jac), tab)
+                lines.extend(pre)
+                islands_jac[var] = jac
     for b in sorted(set(r.bars.values())):
         lines.append(tab + 'Real ' + b + ' = (Real)0;')
     # Outputs: the state update of every state. The direct terms of
@@ -585,9 +623,9 @@ def _vjp_source(cm, precision, islands=True):
             target = r.bars[ex.Derivative(ex.Name(var))]
         else:
             target = r.bars[ex.Name(var)]
-        if islands and precision != DOUBLE_PRECISION and \
-                needs_island(var.rhs(), cm):
-            r.out.extend(_island(r, var.rhs(), target, tab))
+        if var in islands_jac:
+            for bar, J in sorted(islands_jac[var].items()):
+                r.out.append(tab + bar + ' += ' + r.mul(target, J) + ';')
         else:
             r.rev(var.rhs(), target)
     for var in reversed(cm.per_cell_constants):

@@ -507,6 +507,19 @@ __device__ __forceinline__ float mk_powif(float x, int n)
 '''
 
 
+def adjoint_waves_default(cm, precision, adjoint_precision=None):
+    """Waves per SIMD the reverse kernels are compiled for (1: no limit).
+    The reverse step waits on its loads at the start of every cell; one
+    more wave per SIMD hides that latency better than the few registers
+    the limit spills cost, for cell models of up to 12 states with a
+    single-precision VJP (LR1991: 135 registers unbounded, 128 and 6
+    spilled at 4 waves)."""
+    vjp = precision if adjoint_precision is None else adjoint_precision
+    if vjp != DOUBLE_PRECISION and len(cm.states) <= 12:
+        return 4
+    return 1
+
+
 def sheet_cpt(precision):
     """Cells per thread of the persistent sheet driver (4: a tile is 64 x 16
     cells for a 256-thread workgroup)."""
@@ -530,13 +543,16 @@ SHEET_CPT_CHOICES = (1, 2, 4, 8)
 
 def generate_kernel(cm, precision, coupling, block_threads=256,
                     native_maths=False, adjoint=False, tile_cpt=None,
-                    tile_waves=None, adjoint_precision=None):
+                    tile_waves=None, adjoint_precision=None,
+                    adjoint_waves=None):
     """Returns the complete HIP source for hiprtc. ``tile_cpt`` (cells
     per thread of the persistent sheet: tile height 4 x cpt) and
     ``tile_waves`` (waves per SIMD its register budget allows) default to
     :func:`sheet_cpt` and :func:`sheet_waves`. With ``adjoint``, the
     adjoint kernels are appended; ``adjoint_precision`` is the precision of
-    the cell VJP's arithmetic (default: ``precision``)."""
+    the cell VJP's arithmetic (default: ``precision``) and
+    ``adjoint_waves`` the waves per SIMD the reverse kernels' register
+    budget allows (0: no limit; default: :func:`adjoint_waves`)."""
     if block_threads % 64 or not 64 <= block_threads <= 1024:
         raise ValueError('block_threads must be a multiple of 64 in'
                          ' [64, 1024].')
@@ -564,6 +580,9 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_BCELLS ' + str(bcells),
         '#define MK_SCPT ' + str(cpt),
         '#define MK_SWAVES ' + str(waves),
+        '#define MK_ADJ_WAVES ' + str(max(1, int(
+            adjoint_waves_default(cm, precision, adjoint_precision)
+            if adjoint_waves is None else adjoint_waves))),
         'typedef ' + real + ' Real;',
         _read('mk_math.h'),
         _read('mk_args.h'),

@@ -58,34 +58,34 @@ __device__ __forceinline__ double mk_block_sum(double v, double* lds)
  * relative to the arrays, the cell itself at `self`. */
 __device__ __forceinline__ Real mk_adj_couple(
     const MkArgs& a, const Real* __restrict__ V1, const Real* __restrict__ ab,
-    long long c, long long row, long long self, long long ix, long long iy,
-    double& cx, double& cy)
+    int c, int row, int self, int ix, int iy, int nx, int ny, double& cx,
+    double& cy)
 {
     Real lv = (Real)0;
-    (void)row; (void)self; (void)ix; (void)iy; (void)c;
+    (void)row; (void)self; (void)ix; (void)iy; (void)c; (void)nx; (void)ny;
 #if MK_COUPLING == 1
     const Real ac = ab[c], v = V1[c];
     Real ax = (Real)0, ay = (Real)0, xs = (Real)0, ys = (Real)0;
-    if (a.nx > 1) {
+    if (nx > 1) {
         if (ix > 0) { ax += ac - ab[c - 1]; xs += v - V1[c - 1]; }
-        if (ix < a.nx - 1) { ax += ac - ab[c + 1]; xs += v - V1[c + 1]; }
+        if (ix < nx - 1) { ax += ac - ab[c + 1]; xs += v - V1[c + 1]; }
     }
-    if (a.ny > 1) {
-        if (iy > 0) { ay += ac - ab[c - a.nx]; ys += v - V1[c - a.nx]; }
-        if (iy < a.ny - 1) { ay += ac - ab[c + a.nx]; ys += v - V1[c + a.nx]; }
+    if (ny > 1) {
+        if (iy > 0) { ay += ac - ab[c - nx]; ys += v - V1[c - nx]; }
+        if (iy < ny - 1) { ay += ac - ab[c + nx]; ys += v - V1[c + nx]; }
     }
     lv = a.gx * ax + a.gy * ay;
     cx += (double)ac * (double)xs;
     cy += (double)ac * (double)ys;
 #elif MK_COUPLING == 2
     const Real ac = ab[c];
-    if (a.nx > 1) {
+    if (nx > 1) {
         if (ix > 0) lv += a.gx_field[c - iy - 1] * (ac - ab[c - 1]);
-        if (ix < a.nx - 1) lv += a.gx_field[c - iy] * (ac - ab[c + 1]);
+        if (ix < nx - 1) lv += a.gx_field[c - iy] * (ac - ab[c + 1]);
     }
-    if (a.ny > 1) {
-        if (iy > 0) lv += a.gy_field[c - a.nx] * (ac - ab[c - a.nx]);
-        if (iy < a.ny - 1) lv += a.gy_field[c] * (ac - ab[c + a.nx]);
+    if (ny > 1) {
+        if (iy > 0) lv += a.gy_field[c - nx] * (ac - ab[c - nx]);
+        if (iy < ny - 1) lv += a.gy_field[c] * (ac - ab[c + nx]);
     }
 #elif MK_COUPLING == 3
     const Real ac = ab[self], v = V1[self];
@@ -102,8 +102,8 @@ __device__ __forceinline__ Real mk_adj_couple(
 }
 
 /* Loss term dL/dV at cell c for V(y_{s+1}) = v, target row `row`. */
-__device__ __forceinline__ Real mk_adj_loss(const MkAdj& j, long long c,
-                                            Real v, int row)
+__device__ __forceinline__ Real mk_adj_loss(const MkAdj& j, int c, Real v,
+                                            int row)
 {
     if (row < 0) return (Real)0;
     const int q = j.probe_of[c];
@@ -117,16 +117,15 @@ __device__ __forceinline__ Real mk_adj_loss(const MkAdj& j, long long c,
  * as in the forward coupling. */
 __device__ __forceinline__ void mk_adj_cell(
     const MkArgs& a, const MkStep& p, const Real* y, const Real* fv,
-    const Real* __restrict__ V, long long c, long long row, long long self,
-    long long ix, long long iy, const Real* lam, Real* mu, Real& abo,
-    Real* fb)
+    const Real* __restrict__ V, int c, int row, int self, int ix, int iy,
+    int nx, int ny, const Real* lam, Real* mu, Real& abo, Real* fb)
 {
-    (void)row; (void)self;
+    (void)row; (void)self; (void)nx; (void)ny;
 #if MK_COUPLING == 1
-    const Real idiff = mk_stencil(V, c, ix, iy, a.nx, a.ny, a.gx, a.gy);
+    const Real idiff = mk_stencil(V, c, ix, iy, nx, ny, a.gx, a.gy);
 #elif MK_COUPLING == 2
-    const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy, a.nx,
-                                 a.ny);
+    const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy, nx,
+                                 ny);
 #elif MK_COUPLING == 3
     const Real idiff = mk_graph(V, a.csr_ptr, a.csr_idx, a.csr_g, row, self);
 #else
@@ -138,41 +137,46 @@ __device__ __forceinline__ void mk_adj_cell(
 }
 
 /* ---------------------------------------------------------------------- */
-extern "C" __global__ void __launch_bounds__(MK_BT)
+extern "C" __global__ void __launch_bounds__(MK_BT, MK_ADJ_WAVES)
 mk_adj_step(const MkArgs a, const MkAdj j, const int s)
 {
     __shared__ double red[2][MK_BT / 64];
-    const long long c = (long long)blockIdx.x * MK_BT + threadIdx.x;
+    const int c = blockIdx.x * MK_BT + threadIdx.x;
     double cx = 0.0, cy = 0.0;
-    if (c < a.n) {
-        const long long ix = c % a.nx, iy = c / a.nx;
+    if (c < (int)a.n) {
+        const int nx = (int)a.nx, ny = (int)a.ny;
+        const int iy = (int)((unsigned)c / (unsigned)nx), ix = c - iy * nx;
+        const long long np = a.npad;
         Real lam[MK_NS];
+        if (j.mu_in) {
+            const Real* __restrict__ m = j.mu_in + c;
 #pragma unroll
-        for (int i = 0; i < MK_NS; ++i)
-            lam[i] = j.mu_in ? j.mu_in[(long long)i * a.npad + c] : (Real)0;
-        const Real* __restrict__ V1 = j.yn1 + (long long)MK_IVM * a.npad;
+            for (int i = 0; i < MK_NS; ++i) lam[i] = m[i * np];
+        } else {
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i) lam[i] = (Real)0;
+        }
+        const Real* __restrict__ V1 = j.yn1 + MK_IVM * np;
         Real lv = mk_adj_loss(j, c, V1[c], j.target_row);
         if (j.ab_in)
-            lv += mk_adj_couple(a, V1, j.ab_in, c, c, c, ix, iy, cx, cy);
+            lv += mk_adj_couple(a, V1, j.ab_in, c, c, c, ix, iy, nx, ny, cx,
+                                cy);
         lam[MK_IVM] += lv;
         if (j.yn) {
             Real y[MK_NS], mu[MK_NS], fv[MK_NFA], fb[MK_NFA], abo;
+            const Real* __restrict__ yp = j.yn + c;
 #pragma unroll
-            for (int i = 0; i < MK_NS; ++i)
-                y[i] = j.yn[(long long)i * a.npad + c];
+            for (int i = 0; i < MK_NS; ++i) y[i] = yp[i * np];
 #pragma unroll
-            for (int k = 0; k < MK_NF; ++k)
-                fv[k] = a.fields[(long long)k * a.npad + c];
-            mk_adj_cell(a, a.steps[s], y, fv,
-                        j.yn + (long long)MK_IVM * a.npad, c, c, c, ix, iy,
-                        lam, mu, abo, fb);
+            for (int k = 0; k < MK_NF; ++k) fv[k] = a.fields[k * np + c];
+            mk_adj_cell(a, a.steps[s], y, fv, j.yn + MK_IVM * np, c, c, c, ix,
+                        iy, nx, ny, lam, mu, abo, fb);
+            Real* __restrict__ mo = j.mu_out + c;
 #pragma unroll
-            for (int i = 0; i < MK_NS; ++i)
-                j.mu_out[(long long)i * a.npad + c] = mu[i];
+            for (int i = 0; i < MK_NS; ++i) mo[i * np] = mu[i];
             j.ab_out[c] = abo;
 #pragma unroll
-            for (int k = 0; k < MK_NF; ++k)
-                j.gfield[(long long)k * a.npad + c] += (double)fb[k];
+            for (int k = 0; k < MK_NF; ++k) j.gfield[k * np + c] += (double)fb[k];
         }
     }
 #if MK_COUPLING == 1 || MK_COUPLING == 3
@@ -196,7 +200,7 @@ mk_adj_step(const MkArgs a, const MkAdj j, const int s)
  * from one launch to the next. flags: bit 0 = this segment ends the run (no
  * mu or abar of step N yet), bit 1 = this segment starts the run (add the
  * conductance terms of step 0 at the end). */
-extern "C" __global__ void __launch_bounds__(MK_BT)
+extern "C" __global__ void __launch_bounds__(MK_BT, MK_ADJ_WAVES)
 mk_adj_block(const MkArgs a, const MkAdj j, const int s0, const int s1,
              const int flags)
 {
@@ -210,8 +214,9 @@ mk_adj_block(const MkArgs a, const MkAdj j, const int s0, const int s1,
     double cx = 0.0, cy = 0.0;
     /* V(y_{s1}) and abar_{s1} (zero at the end of the run) */
     int cur = 0;
+    const int nx = (int)a.nx, ny = (int)a.ny;
     for (int li = threadIdx.x; li < len; li += MK_BT) {
-        const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+        const int c = a.dom_cells ? a.dom_cells[off + li] : (int)(off + li);
         vsh[0][li] = j.yn[(long long)(s1 - s0) * SN
                           + (long long)MK_IVM * a.npad + c];
         ash[0][li] = last_seg ? (Real)0 : j.ab_out[c];
@@ -221,23 +226,27 @@ mk_adj_block(const MkArgs a, const MkAdj j, const int s0, const int s1,
         const Real* __restrict__ ys = j.yn + (long long)(s - s0) * SN;
         /* V(y_s) of the domain into the other buffer */
         for (int li = threadIdx.x; li < len; li += MK_BT) {
-            const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+            const int c = a.dom_cells ? a.dom_cells[off + li]
+                                      : (int)(off + li);
             vsh[cur ^ 1][li] = ys[(long long)MK_IVM * a.npad + c];
         }
         __syncthreads();
         const MkStep p = a.steps[s];
         const bool fresh = last_seg && s == s1 - 1;
         for (int li = threadIdx.x; li < len; li += MK_BT) {
-            const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
-            const long long ix = c % a.nx, iy = c / a.nx;
+            const int c = a.dom_cells ? a.dom_cells[off + li]
+                                      : (int)(off + li);
+            const int iy = (int)((unsigned)c / (unsigned)nx);
+            const int ix = c - iy * nx;
+            const int row = (int)(off + li);
             Real lam[MK_NS], y[MK_NS], mu[MK_NS], fv[MK_NFA], fb[MK_NFA], abo;
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i)
                 lam[i] = fresh ? (Real)0 : j.mu_out[(long long)i * a.npad + c];
             Real lv = mk_adj_loss(j, c, vsh[cur][li], s);
             if (!fresh)
-                lv += mk_adj_couple(a, vsh[cur], ash[cur], li, off + li, li,
-                                    ix, iy, cx, cy);
+                lv += mk_adj_couple(a, vsh[cur], ash[cur], li, row, li, ix,
+                                    iy, nx, ny, cx, cy);
             lam[MK_IVM] += lv;
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i)
@@ -245,8 +254,8 @@ mk_adj_block(const MkArgs a, const MkAdj j, const int s0, const int s1,
 #pragma unroll
             for (int k = 0; k < MK_NF; ++k)
                 fv[k] = a.fields[(long long)k * a.npad + c];
-            mk_adj_cell(a, p, y, fv, vsh[cur ^ 1], li, off + li, li, ix, iy,
-                        lam, mu, abo, fb);
+            mk_adj_cell(a, p, y, fv, vsh[cur ^ 1], li, row, li, ix, iy, nx,
+                        ny, lam, mu, abo, fb);
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i)
                 j.mu_out[(long long)i * a.npad + c] = mu[i];
@@ -261,11 +270,11 @@ mk_adj_block(const MkArgs a, const MkAdj j, const int s0, const int s1,
     /* Carry abar_{s0} to the previous segment; at the start of the run add
      * the conductance terms of step 0 instead. */
     for (int li = threadIdx.x; li < len; li += MK_BT) {
-        const long long c = a.dom_cells ? a.dom_cells[off + li] : off + li;
+        const int c = a.dom_cells ? a.dom_cells[off + li] : (int)(off + li);
         if (first_seg) {
-            const long long ix = c % a.nx, iy = c / a.nx;
-            (void)mk_adj_couple(a, vsh[cur], ash[cur], li, off + li, li, ix,
-                                iy, cx, cy);
+            const int iy = (int)((unsigned)c / (unsigned)nx);
+            (void)mk_adj_couple(a, vsh[cur], ash[cur], li, (int)(off + li),
+                                li, c - iy * nx, iy, nx, ny, cx, cy);
         } else {
             j.ab_out[c] = ash[cur][li];
         }

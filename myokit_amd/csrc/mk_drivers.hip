@@ -43,21 +43,25 @@ typedef MkStepT<Real> MkStep;
 #define MK_NFA (MK_NF > 0 ? MK_NF : 1)
 #define MK_NQA (MK_NQ > 0 ? MK_NQ : 1)
 
-__device__ __forceinline__ bool mk_is_paced(const MkArgs& a, long long c,
-                                            long long ix, long long iy)
+/* The helpers below take cell indices of type I: long long in the forward
+ * drivers, int in the reverse kernels (at most 2^31 - 1 cells), where the
+ * 64-bit division and address arithmetic would be a fifth of the work. */
+template <typename I>
+__device__ __forceinline__ bool mk_is_paced(const MkArgs& a, I c, I ix, I iy)
 {
     if (a.pace_mode == 0) return true;
     if (a.pace_mode == 1)
-        return ix >= a.px0 && ix < a.px1 && iy >= a.py0 && iy < a.py1;
+        return ix >= (I)a.px0 && ix < (I)a.px1 && iy >= (I)a.py0
+               && iy < (I)a.py1;
     if (a.pace_mode == 2) return a.pace_mask[c] != 0;
     return false;
 }
 
 /* Homogeneous zero-flux stencil (diff_step, openclsim.cl:384-435); V is
  * indexed by cell id, same operation order as the reference. */
+template <typename I>
 __device__ __forceinline__ Real mk_stencil(
-    const Real* __restrict__ V, long long c, long long ix, long long iy,
-    long long nx, long long ny, Real gx, Real gy)
+    const Real* __restrict__ V, I c, I ix, I iy, I nx, I ny, Real gx, Real gy)
 {
     const Real v = V[c];
     Real i;
@@ -77,10 +81,10 @@ __device__ __forceinline__ Real mk_stencil(
 }
 
 /* Conductance-field stencil (diff_hetero, openclsim.cl:452-487). */
+template <typename I>
 __device__ __forceinline__ Real mk_hetero(
     const Real* __restrict__ V, const Real* __restrict__ gxf,
-    const Real* __restrict__ gyf, long long c, long long ix, long long iy,
-    long long nx, long long ny)
+    const Real* __restrict__ gyf, I c, I ix, I iy, I nx, I ny)
 {
     const Real v = V[c];
     Real i = (Real)0;
@@ -99,10 +103,10 @@ __device__ __forceinline__ Real mk_hetero(
  * the connection list: the same additions, in the same order, as the
  * reference's edge scatter (diff_arb_step, openclsim.cl:537-556) when its
  * atomics happen to run in list order. V is indexed by `base + j`. */
+template <typename I>
 __device__ __forceinline__ Real mk_graph(
     const Real* __restrict__ V, const int* __restrict__ ptr,
-    const int* __restrict__ idx, const Real* __restrict__ g,
-    long long row, long long self)
+    const int* __restrict__ idx, const Real* __restrict__ g, I row, I self)
 {
     const Real v = V[self];
     Real i = (Real)0;
@@ -210,14 +214,16 @@ mk_block(const MkArgs a, const int s0, const int s1, Real* const traj)
                 fv[k] = a.fields[(long long)k * a.npad + c];
             const long long ix = c % nx, iy = c / nx;
 #if MK_COUPLING == 1
-            const Real idiff = mk_stencil(vsh[cur], li, ix, iy, nx, ny,
-                                          a.gx, a.gy);
+            const Real idiff = mk_stencil<long long>(vsh[cur], li, ix, iy, nx,
+                                                     ny, a.gx, a.gy);
 #elif MK_COUPLING == 2
-            const Real idiff = mk_hetero(vsh[cur], a.gx_field, a.gy_field,
-                                         li, ix, iy, nx, ny);
+            const Real idiff = mk_hetero<long long>(vsh[cur], a.gx_field,
+                                                    a.gy_field, li, ix, iy,
+                                                    nx, ny);
 #elif MK_COUPLING == 3
-            const Real idiff = mk_graph(vsh[cur], a.csr_ptr, a.csr_idx,
-                                        a.csr_g, off + li, li);
+            const Real idiff = mk_graph<long long>(vsh[cur], a.csr_ptr,
+                                                   a.csr_idx, a.csr_g,
+                                                   off + li, li);
 #else
             const Real idiff = (Real)0;
 #endif
@@ -293,7 +299,8 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
     const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy,
                                  a.nx, a.ny);
 #elif MK_COUPLING == 3
-    const Real idiff = mk_graph(V, a.csr_ptr, a.csr_idx, a.csr_g, c, c);
+    const Real idiff = mk_graph<long long>(V, a.csr_ptr, a.csr_idx, a.csr_g,
+                                           c, c);
 #else
     const Real idiff = (Real)0;
 #endif

@@ -1098,6 +1098,10 @@ struct Sim : SimBase {
             return fail(MKH_E_STATE, "graph adjoint needs the grid or block "
                                      "layout");
         if (K < 1) return fail(MKH_E_ARG, "checkpoint interval must be >= 1");
+        /* the reverse kernels index cells with 32-bit integers */
+        if (n > INT_MAX)
+            return fail(MKH_E_ARG, "the adjoint supports at most 2^31 - 1 "
+                                   "cells");
         Planner pl;
         int f = pl.init(tmin, tmax, dt, events, nev, HUGE_VAL);
         if (f) return pacing_error(f);

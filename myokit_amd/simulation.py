@@ -624,15 +624,17 @@ class SimulationOpenCL:
         return _native.COUPLING_STENCIL
 
     def set_tuning(self, driver=None, block_threads=None, sheet_cpt=None,
-                   sheet_waves=None):
+                   sheet_waves=None, adjoint_waves=None):
         """Launch configuration of the device engine (chosen automatically
         by default; :func:`myokit_amd.tune.autotune` measures and picks
         one). ``driver``: ``'auto'``, ``'cells'``, ``'block'``, ``'grid'``
         or ``'sheet'``; ``block_threads``: threads per workgroup of the
         cells, block and grid kernels (multiple of 64); ``sheet_cpt``:
         cells per thread of the persistent sheet (tile 64 x 4*cpt);
-        ``sheet_waves``: waves per SIMD its register budget allows. Arguments
-        left as None keep their current value."""
+        ``sheet_waves``: waves per SIMD its register budget allows;
+        ``adjoint_waves``: the same for the reverse kernels of
+        :class:`TissueAdjoint` (0: as many as their registers allow).
+        Arguments left as None keep their current value."""
         names = {'auto': -1, 'cells': _native.DRIVER_CELLS,
                  'block': _native.DRIVER_BLOCK, 'grid': _native.DRIVER_GRID,
                  'sheet': _native.DRIVER_SHEET}
@@ -651,6 +653,10 @@ class SimulationOpenCL:
             if not 1 <= int(sheet_waves) <= 8:
                 raise ValueError('sheet_waves must be in [1, 8].')
             self._tuning['sheet_waves'] = int(sheet_waves)
+        if adjoint_waves is not None:
+            if not 0 <= int(adjoint_waves) <= 8:
+                raise ValueError('adjoint_waves must be in [0, 8].')
+            self._tuning['adjoint_waves'] = int(adjoint_waves)
         if driver is not None:
             if driver not in names:
                 raise ValueError('Unknown driver ' + repr(driver) + '.')
@@ -664,7 +670,8 @@ class SimulationOpenCL:
                 'sheet_cpt': self._tuning.get(
                     'sheet_cpt', _kernelgen.sheet_cpt(self._precision)),
                 'sheet_waves': self._tuning.get(
-                    'sheet_waves', _kernelgen.sheet_waves(self._precision))}
+                    'sheet_waves', _kernelgen.sheet_waves(self._precision)),
+                'adjoint_waves': self._tuning.get('adjoint_waves')}
 
     def set_engine_option(self, key, value):
         """Tuning option passed to the device library (``batch_steps``,
@@ -713,7 +720,8 @@ class SimulationOpenCL:
             self._native_math, adjoint=adjoint,
             adjoint_precision=adjoint_precision,
             tile_cpt=tune.get('sheet_cpt'),
-            tile_waves=tune.get('sheet_waves'))
+            tile_waves=tune.get('sheet_waves'),
+            adjoint_waves=tune.get('adjoint_waves'))
         opts = _kernelgen.compile_options(self._precision)
         key = (src, opts, self._ntotal, self._dims)
         if self._engine is not None and self._engine_key == key:

@@ -359,3 +359,32 @@ void run(const float* a, float* y, long n)
     ok = np.isfinite(ref) & (ref != 0)
     assert np.max(np.abs(y[ok] - ref[ok]) / np.abs(ref[ok])) < 4e-7
     assert y[-5] == 0.0 and y[-1] == 1.0
+
+
+def test_island_exp_against_libm(tmp_path):
+    # The double-precision exp of the islands of the single-precision VJP:
+    # 1e-11 relative over the double range, and the limits
+    from myokit_amd._adjointgen import _ISLAND_EXP
+    src = '#include <math.h>\n#define __device__ static\n' \
+          '#define __forceinline__ inline\n' + _ISLAND_EXP + \
+          '''
+void run(const double* x, double* y, long n)
+{
+    for (long i = 0; i < n; ++i) y[i] = mk_exp_isl(x[i]);
+}
+'''
+    lib = _gcc(src, tmp_path / 'expisl')
+    d = ctypes.POINTER(ctypes.c_double)
+    lib.run.argtypes = [d, d, ctypes.c_long]
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(-745, 709, 200000),
+                        rng.uniform(-1e-3, 1e-3, 100000),
+                        [0.0, -800.0, 800.0, np.inf, -np.inf]])
+    y = np.zeros_like(x)
+    lib.run(x.ctypes.data_as(d), y.ctypes.data_as(d), len(x))
+    with np.errstate(over='ignore'):
+        ref = np.exp(x)
+    ok = np.isfinite(ref) & (ref >= 2.0 ** -1022)
+    assert np.max(np.abs(y[ok] - ref[ok]) / ref[ok]) < 1e-11
+    assert y[-5] == 1.0 and y[-4] == 0.0 and np.isinf(y[-3])
+    assert np.isinf(y[-2]) and y[-1] == 0.0

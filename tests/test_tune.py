@@ -50,11 +50,13 @@ def test_set_tuning_validation():
         s.set_tuning(sheet_waves=0)
     with pytest.raises(ValueError):
         s.set_tuning(driver='fast')
+    with pytest.raises(ValueError):
+        s.set_tuning(adjoint_waves=9)
     s.set_tuning(driver='grid', block_threads=128, sheet_cpt=8,
-                 sheet_waves=3)
+                 sheet_waves=3, adjoint_waves=4)
     t = s.tuning()
     assert t == {'driver': 'grid', 'block_threads': 128, 'sheet_cpt': 8,
-                 'sheet_waves': 3}
+                 'sheet_waves': 3, 'adjoint_waves': 4}
 
 
 def test_cache_round_trip(tmp_path, monkeypatch):
