@@ -500,6 +500,20 @@ def _island(r, e, k, tab):
     r64._inf = r._inf
     for key in list(r64.bars):
         r64.bars[key] = 'isl_' + r.bars[key]
+    # each distinct exp once (inside the branches of a piecewise the
+    # compiler would not merge repeated evaluations)
+    exps = []
+
+    def find(x):
+        for y in x:
+            find(y)
+        if isinstance(x, ex.Exp) and x.code() not in r64.w.subst:
+            name = 'ie' + str(k) + '_' + str(len(exps))
+            exps.append('const double ' + name + ' = '
+                        + r64.w.ex(x) + ';')
+            r64.w.subst[x.code()] = name
+    find(e)
+    r64.out.extend(tab + line for line in exps)
     r64.rev(e, '1.0')
     r._ntmp = r64._ntmp
     body = '\n'.join(r64.out)

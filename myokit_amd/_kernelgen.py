@@ -92,6 +92,8 @@ class HipExpressionWriter:
         self.native = bool(native_maths)
         self.names = {}
         self.constants = {}
+        # sub-expression code -> C name of a local holding its value
+        self.subst = {}
 
     def fn(self, name):
         d, f, fn = _FUNCS[name]
@@ -100,6 +102,10 @@ class HipExpressionWriter:
         return fn if self.native else f
 
     def ex(self, e):
+        if self.subst:
+            name = self.subst.get(e.code())
+            if name is not None:
+                return name
         m = getattr(self, '_ex_' + type(e).__name__)
         return m(e)
 
