@@ -107,6 +107,21 @@ int mkh_sim_get_state(void* sim, double* state, long long n);
  * reference, whose state lives in host lists between runs. */
 int mkh_sim_copy_state(void* dst, void* src);
 
+/* FiberTissueSimulation (_sim/fiber_tissue.py:17, its sim_step
+ * _sim/fiber_tissue.c:1003-1160 and the junction kernel
+ * diff_step_fiber_tissue, _sim/openclsim.cl:601-628). `fiber` and `tissue`
+ * are two simulations of one precision with stencil coupling and the grid
+ * driver; fiber row k's last cell connects to tissue cell (ctx, cty + k)
+ * with conductance gft. Both are initialised with mkh_sim_init for the
+ * same run (and given their log buffers); each mkh_ft_step then advances
+ * both by one batch in lockstep, like mkh_sim_step (status 0 more, 1 done;
+ * MKH_E_NAN when the NaN check of either part halts the run, both logs
+ * then ending at the same row). */
+int mkh_ft_create(void* fiber, void* tissue, long long ctx, long long cty,
+                  double gft, void** ft);
+int mkh_ft_step(void* ft, double* t, int* status);
+int mkh_ft_destroy(void* ft);
+
 /* Per-cell fields, cell-major / field-minor (openclsim.py:1082-1089). */
 int mkh_sim_set_fields(void* sim, const double* data, long long n);
 

@@ -12,7 +12,8 @@ from ._err import (  # noqa
     CyclicalDependencyError, DataBlockReadError, DataLogReadError,
     DeviceError, FindNanError,
     IllegalReferenceError,
-    IntegrityError, MyokitError, NumericalError, ParseError,
+    IntegrityError, MissingRhsError, MyokitError, NumericalError,
+    ParseError,
     ProtocolEventError, SimulationCancelledError, SimulationError,
     SimultaneousProtocolEventError)
 from ._datablock import DataBlock1d, DataBlock2d  # noqa
@@ -32,5 +33,6 @@ from ._progress import ProgressPrinter, ProgressReporter, Timeout  # noqa
 from ._protocol import PacingSystem, Protocol, ProtocolEvent  # noqa
 from . import pacing  # noqa
 from .simulation import SimulationOpenCL  # noqa
+from .fiber_tissue import FiberTissueSimulation  # noqa
 
 __version__ = '0.1.0'

@@ -31,6 +31,10 @@ class IntegrityError(MyokitError):
     """Raised when a model is invalid (unresolved names, cycles, ...)."""
 
 
+class MissingRhsError(IntegrityError):
+    """Raised when a variable has no right-hand side."""
+
+
 class IllegalReferenceError(IntegrityError):
     """Raised when a name cannot be resolved in its scope."""
 

@@ -18,7 +18,7 @@ from collections import OrderedDict
 from . import _expressions as ex
 from ._err import (
     CyclicalDependencyError, IllegalReferenceError, IntegrityError,
-    NumericalError)
+    NumericalError, MissingRhsError)
 
 _NAME = re.compile(r'^[a-zA-Z_]\w*$')
 
@@ -151,7 +151,15 @@ class Variable(_VarOwner):
         return self._rhs
 
     def set_rhs(self, rhs):
-        if rhs is not None and not isinstance(rhs, ex.Expression):
+        """Sets the right-hand side: an Expression, a number, or the text
+        of an expression (names resolved when the model is validated)."""
+        if isinstance(rhs, str):
+            try:
+                rhs = ex.Number(float(rhs))
+            except ValueError:
+                from ._parser import parse_expression
+                rhs = parse_expression(rhs)
+        elif rhs is not None and not isinstance(rhs, ex.Expression):
             rhs = ex.Number(rhs)
         self._rhs = rhs
         self.model()._invalidate()
@@ -663,7 +671,7 @@ class _Analysis:
         # Resolve string names
         for v in allvars:
             if v._rhs is None:
-                raise IntegrityError(
+                raise MissingRhsError(
                     'No RHS set for <' + v.qname() + '>.')
             v._rhs = _resolve(v, v._rhs)
         for s in model._states:

@@ -84,6 +84,10 @@ SIGNATURES = {
                          _pd, _pd, _pd]),
     'mkh_adj_timing': (_i, [_vp, _pd]),
     'mkh_sim_copy_state': (_i, [_vp, _vp]),
+    'mkh_ft_create': (_i, [_vp, _vp, _ll, _ll, _d,
+                           ctypes.POINTER(ctypes.c_void_p)]),
+    'mkh_ft_step': (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_int)]),
+    'mkh_ft_destroy': (_i, [_vp]),
     'mkh_host_alloc': (_i, [_ll, ctypes.POINTER(ctypes.c_void_p)]),
     'mkh_host_free': (_i, [_vp]),
     'mkh_adj_trace': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll]),

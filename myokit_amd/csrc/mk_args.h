@@ -32,6 +32,8 @@ struct MkArgsT {
     const R* csr_g;             /* graph: conductance per entry */
     const R* gx_field;          /* hetero: (ny, nx - 1) */
     const R* gy_field;          /* hetero: (ny - 1, nx) */
+    const R* idiff_extra;       /* grid driver: current added to idiff where
+                                   nonzero (fiber-tissue junction), or NULL */
     const long long* dom_off;   /* block driver: domain offsets into dom_cells */
     const int* dom_cells;       /* block driver: cell ids, grouped by domain
                                    (NULL = identity order) */

@@ -294,16 +294,23 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
     for (int k = 0; k < MK_NF; ++k) fv[k] = a.fields[(long long)k * a.npad + c];
     const Real* __restrict__ V = src + (long long)MK_IVM * a.npad;
 #if MK_COUPLING == 1
-    const Real idiff = mk_stencil(V, c, ix, iy, a.nx, a.ny, a.gx, a.gy);
+    Real idiff = mk_stencil(V, c, ix, iy, a.nx, a.ny, a.gx, a.gy);
 #elif MK_COUPLING == 2
-    const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy,
+    Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy,
                                  a.nx, a.ny);
 #elif MK_COUPLING == 3
-    const Real idiff = mk_graph<long long>(V, a.csr_ptr, a.csr_idx, a.csr_g,
+    Real idiff = mk_graph<long long>(V, a.csr_ptr, a.csr_idx, a.csr_g,
                                            c, c);
 #else
-    const Real idiff = (Real)0;
+    Real idiff = (Real)0;
 #endif
+    /* junction current of the fiber-tissue driver, added after the stencil
+     * as the reference adds it (diff_step_fiber_tissue, openclsim.cl:601-
+     * 628); zero elsewhere, where adding nothing keeps idiff bit for bit */
+    if (a.idiff_extra) {
+        const Real e = a.idiff_extra[c];
+        if (e != (Real)0) idiff += e;
+    }
     const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace : (Real)0;
     if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
         __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
@@ -550,4 +557,26 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
     if (threadIdx.x == 0)
         __hip_atomic_store(sh.cnt + tile, sh.g0 + (s1 - s0), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* ------------------------------------------------------------------------
+ * Fiber-tissue junction (diff_step_fiber_tissue, openclsim.cl:601-628):
+ * fiber row k's last cell (nfx - 1, k) connects to tissue cell (ctx,
+ * cty + k) with conductance gft. The current gft (Vf - Vt) is written for
+ * the fiber cell and its negative for the tissue cell, into arrays that
+ * are zero elsewhere; the grid driver adds them after its stencil.
+ * ---------------------------------------------------------------------- */
+extern "C" __global__ void __launch_bounds__(64)
+mk_junction(const Real* __restrict__ vf, const Real* __restrict__ vt,
+            Real* __restrict__ jf, Real* __restrict__ jt, const long long nfx,
+            const long long nfy, const long long ntx, const long long ctx,
+            const long long cty, const Real gft)
+{
+    const long long k = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (k >= nfy) return;
+    const long long iff = (nfx - 1) + k * nfx;
+    const long long ift = ctx + (cty + k) * ntx;
+    const Real i = gft * (vf[iff] - vt[ift]);
+    jf[iff] = i;
+    jt[ift] = -i;
 }

@@ -60,7 +60,7 @@ struct Program {
     int device = -1;
     hipModule_t module = nullptr;
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
-    hipFunction_t f_sheet = nullptr;
+    hipFunction_t f_sheet = nullptr, f_junction = nullptr;
     hipFunction_t f_adj_step = nullptr, f_adj_block = nullptr,
                   f_adj_probe = nullptr;
     std::vector<char> code;
@@ -812,74 +812,158 @@ struct Sim : SimBase {
         return 0;
     }
 
+    /* A batch of planned steps. step() plans, launches and finishes one
+     * batch for this simulation; the fiber-tissue driver interleaves the
+     * launches of two simulations step by step. */
+    struct Batch {
+        long long ns = 0, nr = 0;
+        std::vector<double> tlog, plog;
+        size_t nev = 0;
+    };
+
+    int plan_batch(Batch& b)
+    {
+        b = Batch();
+        row_t_after.clear();
+        StepRec r;
+        while (!plan.done && b.ns < batch_steps && b.nr < log_rows_cap_run) {
+            int f = plan.next(r);
+            if (f) return pacing_error(f);
+            Step& st = h_steps[b.ns++];
+            st.t = (R)r.t;
+            st.dt = (R)r.dt;
+            st.pace = (R)r.pace;
+            st.row = -1;
+            if (r.log) {
+                st.row = (int)b.nr++;
+                row_t_after.push_back(r.t_after);
+                b.tlog.push_back((double)(R)r.t);
+                b.plog.push_back((double)(R)r.pace);
+            }
+        }
+        if (b.ns > 0)
+            MKH_HIP(hipMemcpyAsync(steps.p, h_steps, b.ns * sizeof(Step),
+                                   hipMemcpyHostToDevice, stream));
+        return 0;
+    }
+
+    /* After the launch that ends with step s: if s logs a row, copy the row
+     * to the caller's buffer on the copy stream. */
+    int emit_row(const Batch& b, long long s)
+    {
+        const int row = h_steps[s].row;
+        if (!(b.nr > 0 && ncols > 0 && dst_values) || row < 0) return 0;
+        hipEvent_t e = row_ev[row % 16];
+        MKH_HIP(hipEventRecord(e, stream));
+        MKH_HIP(hipStreamWaitEvent(copy_stream, e, 0));
+        MKH_HIP(hipMemcpyAsync(
+            dst_values + (size_t)(rows_done + row) * ncols,
+            logbuf.p + (size_t)row * ncols, (size_t)ncols * sizeof(R),
+            hipMemcpyDeviceToHost, copy_stream));
+        return 0;
+    }
+
+    unsigned grid_blocks() const
+    {
+        const unsigned bt = (unsigned)cfg.block_threads;
+        if ((cfg.coupling == MKH_COUPLING_STENCIL ||
+             cfg.coupling == MKH_COUPLING_HETERO) && cfg.ny > 1) {
+            long long tx = (cfg.nx + 63) / 64;
+            long long ty = (cfg.ny + (bt / 64) - 1) / (bt / 64);
+            return (unsigned)(tx * ty);
+        }
+        return (unsigned)((n + bt - 1) / bt);
+    }
+
+    /* Step s of the batch with the grid driver; `extra` (or null) is a
+     * current per cell added to the diffusion current where nonzero. */
+    int grid_launch(Batch& b, long long s, const R* extra)
+    {
+        Args a = base_args();
+        a.idiff_extra = extra;
+        R* src = state[cur].p;
+        R* dst = state[cur ^ 1].p;
+        int si = (int)s;
+        void* params[] = {&a, &src, &dst, &si};
+        int e = launch(prog->f_grid, grid_blocks(), params, b.nev);
+        if (e) return e;
+        cur ^= 1;
+        return emit_row(b, s);
+    }
+
+    /* Waits for the batch; copies times and paces of its rows. Returns the
+     * number of valid rows (all, or up to the one where the NaN check
+     * halted the run) through `valid`. */
+    int finish_batch(Batch& b, long long& valid, int& halted)
+    {
+        cell_steps += b.ns * n;
+        int h = -1;
+        MKH_HIP(hipMemcpyAsync(&h, halt.p, sizeof(int), hipMemcpyDeviceToHost,
+                               stream));
+        MKH_HIP(hipStreamSynchronize(stream));
+        MKH_HIP(hipStreamSynchronize(copy_stream));
+        if (driver == MKH_DRIVER_SHEET) {
+            int e = 0;
+            MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
+                              hipMemcpyDeviceToHost));
+            if (e) {
+                running = false;
+                return fail(MKH_E_HIP, "sheet driver: a neighbour exchange "
+                                       "timed out");
+            }
+        }
+        valid = b.nr;
+        halted = h;
+        if (h >= 0) valid = h + 1;
+        if (time_kernels) {
+            for (size_t k = 0; k + 1 < b.nev; k += 2) {
+                float ms = 0;
+                MKH_HIP(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
+                kernel_ms += ms;
+            }
+        }
+        return 0;
+    }
+
+    /* Records `valid` rows of the batch as done and sets the outputs of
+     * step(): the time reached, and status 0 (more), 1 (done) or 2 (NaN). */
+    int close_batch(const Batch& b, long long valid, int halted,
+                    double* tout, int* status)
+    {
+        for (long long k = 0; k < valid; ++k) {
+            if (dst_times) dst_times[rows_done + k] = b.tlog[k];
+            if (dst_paces) dst_paces[rows_done + k] = b.plog[k];
+        }
+        rows_done += valid;
+        if (halted >= 0) {
+            *tout = row_t_after[halted];
+            *status = 2;
+            running = false;
+            return fail(MKH_E_NAN, "Encountered nan in simulation.");
+        }
+        *tout = plan.t;
+        *status = plan.done ? 1 : 0;
+        if (plan.done) running = false;
+        return 0;
+    }
+
     int step(double* tout, int* status) override
     {
         if (!running) return fail(MKH_E_STATE, "simulation not initialised");
-        /* Plan one batch */
-        long long ns = 0, nr = 0;
-        row_t_after.clear();
-        std::vector<double> tlog, plog;
-        StepRec r;
-        while (!plan.done && ns < batch_steps && nr < log_rows_cap_run) {
-            int f = plan.next(r);
-            if (f) return pacing_error(f);
-            Step& s = h_steps[ns++];
-            s.t = (R)r.t;
-            s.dt = (R)r.dt;
-            s.pace = (R)r.pace;
-            s.row = -1;
-            if (r.log) {
-                s.row = (int)nr++;
-                row_t_after.push_back(r.t_after);
-                tlog.push_back((double)(R)r.t);
-                plog.push_back((double)(R)r.pace);
-            }
-        }
+        Batch b;
+        int e = plan_batch(b);
+        if (e) return e;
+        const long long ns = b.ns;
         if (ns == 0) {
             *tout = plan.t;
             *status = 1;
             return 0;
         }
-        MKH_HIP(hipMemcpyAsync(steps.p, h_steps, ns * sizeof(Step),
-                               hipMemcpyHostToDevice, stream));
-        /* After the launch that ends with step s: if s logs a row, copy the
-         * row to the caller's buffer on the copy stream. */
-        const bool copy_rows = nr > 0 && ncols > 0 && dst_values;
-        auto emit_row = [&](long long s) -> int {
-            const int row = h_steps[s].row;
-            if (!copy_rows || row < 0) return 0;
-            hipEvent_t e = row_ev[row % 16];
-            MKH_HIP(hipEventRecord(e, stream));
-            MKH_HIP(hipStreamWaitEvent(copy_stream, e, 0));
-            MKH_HIP(hipMemcpyAsync(
-                dst_values + (size_t)(rows_done + row) * ncols,
-                logbuf.p + (size_t)row * ncols, (size_t)ncols * sizeof(R),
-                hipMemcpyDeviceToHost, copy_stream));
-            return 0;
-        };
         /* Launch */
-        size_t nev = 0;
         const unsigned bt = (unsigned)cfg.block_threads;
         if (driver == MKH_DRIVER_GRID) {
-            unsigned grid;
-            if ((cfg.coupling == MKH_COUPLING_STENCIL ||
-                 cfg.coupling == MKH_COUPLING_HETERO) && cfg.ny > 1) {
-                long long tx = (cfg.nx + 63) / 64;
-                long long ty = (cfg.ny + (bt / 64) - 1) / (bt / 64);
-                grid = (unsigned)(tx * ty);
-            } else {
-                grid = (unsigned)((n + bt - 1) / bt);
-            }
             for (long long s = 0; s < ns; ++s) {
-                Args a = base_args();
-                R* src = state[cur].p;
-                R* dst = state[cur ^ 1].p;
-                int si = (int)s;
-                void* params[] = {&a, &src, &dst, &si};
-                int e = launch(prog->f_grid, grid, params, nev);
-                if (e) return e;
-                cur ^= 1;
-                e = emit_row(s);
+                e = grid_launch(b, s, nullptr);
                 if (e) return e;
             }
         } else if (driver == MKH_DRIVER_SHEET) {
@@ -894,19 +978,19 @@ struct Sim : SimBase {
                     sh.g0 = (int)sh_steps;
                     void* params[] = {&a, &sh, &i0, &i1};
                     if (time_kernels)
-                        MKH_HIP(hipEventRecord(event(nev), stream));
+                        MKH_HIP(hipEventRecord(event(b.nev), stream));
                     MKH_HIP(hipModuleLaunchCooperativeKernel(
                         prog->f_sheet, ntiles, 1, 1, 256, 1, 1, 0, stream,
                         params));
                     if (time_kernels) {
-                        MKH_HIP(hipEventRecord(event(nev + 1), stream));
-                        nev += 2;
+                        MKH_HIP(hipEventRecord(event(b.nev + 1), stream));
+                        b.nev += 2;
                     }
                     launches++;
                     sh_steps += i1 - i0;
                     cur ^= 1;
                     s0 = s + 1;
-                    int e = emit_row(s);
+                    e = emit_row(b, s);
                     if (e) return e;
                 }
             }
@@ -923,55 +1007,19 @@ struct Sim : SimBase {
                 if (h_steps[s].row >= 0 || s == ns - 1) {
                     int i0 = (int)s0, i1 = (int)(s + 1);
                     void* params[] = {&a, &i0, &i1, &no_traj};
-                    int e = launch(f, grid, params, nev);
+                    e = launch(f, grid, params, b.nev);
                     if (e) return e;
                     s0 = s + 1;
-                    e = emit_row(s);
+                    e = emit_row(b, s);
                     if (e) return e;
                 }
             }
         }
-        cell_steps += ns * n;
-        /* Bring back the log and the halt flag */
-        int h = -1;
-        MKH_HIP(hipMemcpyAsync(&h, halt.p, sizeof(int), hipMemcpyDeviceToHost,
-                               stream));
-        MKH_HIP(hipStreamSynchronize(stream));
-        MKH_HIP(hipStreamSynchronize(copy_stream));
-        if (driver == MKH_DRIVER_SHEET) {
-            int e = 0;
-            MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
-                              hipMemcpyDeviceToHost));
-            if (e) {
-                running = false;
-                return fail(MKH_E_HIP, "sheet driver: a neighbour exchange "
-                                       "timed out");
-            }
-        }
-        long long valid = nr;
-        if (h >= 0) valid = h + 1;
-        for (long long k = 0; k < valid; ++k) {
-            if (dst_times) dst_times[rows_done + k] = tlog[k];
-            if (dst_paces) dst_paces[rows_done + k] = plog[k];
-        }
-        rows_done += valid;
-        if (time_kernels) {
-            for (size_t k = 0; k + 1 < nev; k += 2) {
-                float ms = 0;
-                MKH_HIP(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
-                kernel_ms += ms;
-            }
-        }
-        if (h >= 0) {
-            *tout = row_t_after[h];
-            *status = 2;
-            running = false;
-            return fail(MKH_E_NAN, "Encountered nan in simulation.");
-        }
-        *tout = plan.t;
-        *status = plan.done ? 1 : 0;
-        if (plan.done) running = false;
-        return 0;
+        long long valid = 0;
+        int halted = -1;
+        e = finish_batch(b, valid, halted);
+        if (e) return e;
+        return close_batch(b, valid, halted, tout, status);
     }
 
     int clean() override
@@ -1478,6 +1526,123 @@ struct Sim : SimBase {
 /* -------------------------------------------------------------------------
  * C ABI
  * ---------------------------------------------------------------------- */
+/* -------------------------------------------------------------------------
+ * Fiber-tissue driver (FiberTissueSimulation's sim_step,
+ * _sim/fiber_tissue.c:1003-1160): two simulations, each with its own model
+ * and kernel program, advanced in lockstep on one stream with the grid
+ * driver. Per step: the junction current from both potentials at t
+ * (mk_junction), then the fiber step and the tissue step, each adding its
+ * junction current after its stencil. Both simulations were initialised
+ * with the same run, so their planners give the same steps.
+ * ---------------------------------------------------------------------- */
+struct FtBase {
+    virtual ~FtBase() {}
+    virtual int step(double*, int*) = 0;
+};
+
+template <typename R>
+struct FtSim : FtBase {
+    Sim<R>* f = nullptr;
+    Sim<R>* t = nullptr;
+    DevBuf<R> jf, jt;
+    long long nfx = 0, nfy = 0, ntx = 0, ctx = 0, cty = 0;
+    R gft = 0;
+
+    int create(Sim<R>* fs, Sim<R>* ts, long long ctx_, long long cty_,
+               double g)
+    {
+        f = fs;
+        t = ts;
+        if (f->cfg.coupling != MKH_COUPLING_STENCIL ||
+            t->cfg.coupling != MKH_COUPLING_STENCIL)
+            return fail(MKH_E_ARG, "fiber and tissue need stencil coupling");
+        if (f->cfg.device != t->cfg.device)
+            return fail(MKH_E_ARG, "fiber and tissue on different devices");
+        nfx = f->cfg.nx;
+        nfy = f->cfg.ny;
+        ntx = t->cfg.nx;
+        ctx = ctx_;
+        cty = cty_;
+        gft = (R)g;
+        if (ctx < 0 || ctx >= ntx || cty < 0 || cty + nfy > t->cfg.ny)
+            return fail(MKH_E_ARG, "junction outside the tissue");
+        MKH_HIP(hipSetDevice(f->cfg.device));
+        MKH_HIP(jf.alloc((size_t)f->npad));
+        MKH_HIP(jt.alloc((size_t)t->npad));
+        MKH_HIP(hipMemset(jf.p, 0, jf.n * sizeof(R)));
+        MKH_HIP(hipMemset(jt.p, 0, jt.n * sizeof(R)));
+        return 0;
+    }
+
+    int step(double* tout, int* status) override
+    {
+        if (!f->running || !t->running)
+            return fail(MKH_E_STATE, "simulation not initialised");
+        if (f->driver != MKH_DRIVER_GRID || t->driver != MKH_DRIVER_GRID)
+            return fail(MKH_E_STATE, "fiber-tissue steps need the grid "
+                                     "driver");
+        MKH_HIP(hipSetDevice(f->cfg.device));
+        /* the tissue works on the fiber's stream for this batch */
+        hipStream_t own = t->stream;
+        MKH_HIP(hipStreamSynchronize(own));
+        MKH_HIP(hipStreamSynchronize(f->stream));
+        t->stream = f->stream;
+        struct Restore {
+            Sim<R>* s;
+            hipStream_t st;
+            ~Restore() { s->stream = st; }
+        } restore{t, own};
+        const long long cap = std::min(f->log_rows_cap_run,
+                                       t->log_rows_cap_run);
+        f->log_rows_cap_run = t->log_rows_cap_run = cap;
+        f->batch_steps = t->batch_steps = std::min(f->batch_steps,
+                                                   t->batch_steps);
+        typename Sim<R>::Batch bf, bt;
+        int e = f->plan_batch(bf);
+        if (e) return e;
+        e = t->plan_batch(bt);
+        if (e) return e;
+        if (bf.ns != bt.ns || bf.nr != bt.nr)
+            return fail(MKH_E_STATE, "fiber and tissue runs differ");
+        if (bf.ns == 0) {
+            *tout = f->plan.t;
+            *status = 1;
+            return 0;
+        }
+        const unsigned jgrid = (unsigned)((nfy + 63) / 64);
+        for (long long s = 0; s < bf.ns; ++s) {
+            const R* vf = f->state[f->cur].p + (long long)f->cfg.i_vm * f->npad;
+            const R* vt = t->state[t->cur].p + (long long)t->cfg.i_vm * t->npad;
+            R* pjf = jf.p;
+            R* pjt = jt.p;
+            void* params[] = {&vf, &vt, &pjf, &pjt, &nfx, &nfy, &ntx, &ctx,
+                              &cty, &gft};
+            MKH_HIP(hipModuleLaunchKernel(f->prog->f_junction, jgrid, 1, 1,
+                                          64, 1, 1, 0, f->stream, params,
+                                          nullptr));
+            e = f->grid_launch(bf, s, jf.p);
+            if (e) return e;
+            e = t->grid_launch(bt, s, jt.p);
+            if (e) return e;
+        }
+        long long vf_rows = 0, vt_rows = 0;
+        int hf = -1, ht = -1;
+        e = f->finish_batch(bf, vf_rows, hf);
+        if (e) return e;
+        e = t->finish_batch(bt, vt_rows, ht);
+        if (e) return e;
+        /* a NaN in either part ends both at the same row (halt_sim) */
+        int h = hf;
+        if (ht >= 0 && (h < 0 || ht < h)) h = ht;
+        const long long valid = h >= 0 ? h + 1 : bf.nr;
+        double t2 = 0;
+        int s2 = 0;
+        e = t->close_batch(bt, valid, h, &t2, &s2);
+        const int ef = f->close_batch(bf, valid, h, tout, status);
+        return ef ? ef : e;
+    }
+};
+
 extern "C" {
 
 const char* mkh_version(void) { return "myokit_amd 0.1"; }
@@ -1586,6 +1751,8 @@ int mkh_program_build(const char* source, const char* options,
         MKH_HIP(hipModuleGetFunction(&prog->f_block, prog->module, "mk_block"));
         MKH_HIP(hipModuleGetFunction(&prog->f_grid, prog->module, "mk_grid"));
         MKH_HIP(hipModuleGetFunction(&prog->f_sheet, prog->module, "mk_sheet"));
+        MKH_HIP(hipModuleGetFunction(&prog->f_junction, prog->module,
+                                     "mk_junction"));
         /* present only in kernels generated with gradients */
         if (hipModuleGetFunction(&prog->f_adj_step, prog->module,
                                  "mk_adj_step") != hipSuccess ||
@@ -1670,6 +1837,48 @@ int mkh_sim_copy_state(void* dst, void* src)
     if (!dst || !src) return fail(MKH_E_ARG, "null simulation");
     return static_cast<SimBase*>(dst)->copy_state_from(
         static_cast<SimBase*>(src));
+}
+
+int mkh_ft_create(void* fiber, void* tissue, long long ctx, long long cty,
+                  double gft, void** ft)
+{
+    if (!fiber || !tissue || !ft) return fail(MKH_E_ARG, "null argument");
+    *ft = nullptr;
+    SimBase* a = static_cast<SimBase*>(fiber);
+    SimBase* b = static_cast<SimBase*>(tissue);
+    if (a->cfg.precision != b->cfg.precision)
+        return fail(MKH_E_ARG, "fiber and tissue differ in precision");
+    int r;
+    FtBase* h;
+    if (a->cfg.precision == 64) {
+        FtSim<double>* d = new FtSim<double>();
+        r = d->create(static_cast<Sim<double>*>(a),
+                      static_cast<Sim<double>*>(b), ctx, cty, gft);
+        h = d;
+    } else {
+        FtSim<float>* d = new FtSim<float>();
+        r = d->create(static_cast<Sim<float>*>(a),
+                      static_cast<Sim<float>*>(b), ctx, cty, gft);
+        h = d;
+    }
+    if (r) {
+        delete h;
+        return r;
+    }
+    *ft = h;
+    return 0;
+}
+
+int mkh_ft_step(void* ft, double* t, int* status)
+{
+    if (!ft || !t || !status) return fail(MKH_E_ARG, "null argument");
+    return static_cast<FtBase*>(ft)->step(t, status);
+}
+
+int mkh_ft_destroy(void* ft)
+{
+    delete static_cast<FtBase*>(ft);
+    return 0;
 }
 
 int mkh_sim_set_fields(void* sim, const double* d, long long n)

@@ -921,7 +921,48 @@ class SimulationOpenCL:
                  log_interval, progress=None, msg=''):
         """Runs from tmin to tmax on the device. Returns (values[rows, ncols],
         times, paces, arithmetic_error)."""
+        run = self._exec_begin(tmin, tmax, quantities, colmap, ncols,
+                               logging_states, log_interval, progress)
+        L = _native.lib()
+        h = run.eng.h
         duration = tmax - tmin
+        t = ctypes.c_double(tmin)
+        status = ctypes.c_int(0)
+        arithmetic_error = False
+        ok = False
+        try:
+            if progress:
+                with progress.job(msg):
+                    r = 1.0 / duration if duration != 0 else 1
+                    while status.value == 0:
+                        code = L.mkh_sim_step(h, ctypes.byref(t),
+                                              ctypes.byref(status))
+                        if code == _native.MKH_E_NAN:
+                            arithmetic_error = True
+                            break
+                        self._check_step(code)
+                        if not progress.update(min((t.value - tmin) * r, 1)):
+                            raise SimulationCancelledError()
+            else:
+                while status.value == 0:
+                    code = L.mkh_sim_step(h, ctypes.byref(t),
+                                          ctypes.byref(status))
+                    if code == _native.MKH_E_NAN:
+                        arithmetic_error = True
+                        break
+                    self._check_step(code)
+            ok = True
+        finally:
+            values, times, paces = self._exec_end(run, ok)
+        return values, times, paces, arithmetic_error
+
+    def _exec_begin(self, tmin, tmax, quantities, colmap, ncols,
+                    logging_states, log_interval, progress=None):
+        """Prepares the engine for a run from tmin to tmax: logging, the
+        step plan and the log buffers. Steps are then taken by mkh_sim_step
+        (or, for two simulations in lockstep, mkh_ft_step) and the run is
+        closed by :meth:`_exec_end`."""
+        import types
         eng, cm = self._get_engine(quantities)
         if progress is not None and self._dev_newer:
             # Keep a host copy: a cancelled run leaves the state unchanged
@@ -959,47 +1000,28 @@ class SimulationOpenCL:
             h, values.ctypes.data_as(ctypes.c_void_p),
             _native.ptr(times, ctypes.c_double),
             _native.ptr(paces, ctypes.c_double)), 'mkh_sim_set_log')
-        t = ctypes.c_double(tmin)
-        status = ctypes.c_int(0)
-        arithmetic_error = False
-        t0 = _time.perf_counter()
-        ok = False
-        try:
-            if progress:
-                with progress.job(msg):
-                    r = 1.0 / duration if duration != 0 else 1
-                    while status.value == 0:
-                        code = L.mkh_sim_step(h, ctypes.byref(t),
-                                              ctypes.byref(status))
-                        if code == _native.MKH_E_NAN:
-                            arithmetic_error = True
-                            break
-                        self._check_step(code)
-                        if not progress.update(min((t.value - tmin) * r, 1)):
-                            raise SimulationCancelledError()
-            else:
-                while status.value == 0:
-                    code = L.mkh_sim_step(h, ctypes.byref(t),
-                                          ctypes.byref(status))
-                    if code == _native.MKH_E_NAN:
-                        arithmetic_error = True
-                        break
-                    self._check_step(code)
-            ok = True
-        finally:
-            L.mkh_sim_clean(h)
-            self.last_run_stats = {
-                'engine_seconds': _time.perf_counter() - t0,
-                'cell_steps': None}
-            if ok:
-                self._dev_newer = True
-            else:
-                # Device state is partly advanced: next run uploads the host
-                self._host_dirty = True
+        return types.SimpleNamespace(
+            eng=eng, values=values, times=times, paces=paces,
+            t0=_time.perf_counter())
+
+    def _exec_end(self, run, ok):
+        """Closes a run of :meth:`_exec_begin`; returns the logged rows
+        (values, times, paces)."""
+        L = _native.lib()
+        h = run.eng.h
+        L.mkh_sim_clean(h)
+        self.last_run_stats = {
+            'engine_seconds': _time.perf_counter() - run.t0,
+            'cell_steps': None}
+        if ok:
+            self._dev_newer = True
+        else:
+            # Device state is partly advanced: next run uploads the host
+            self._host_dirty = True
         rows = ctypes.c_longlong(0)
         L.mkh_sim_log_rows(h, ctypes.byref(rows))
         rows = rows.value
-        return values[:rows], times[:rows], paces[:rows], arithmetic_error
+        return run.values[:rows], run.times[:rows], run.paces[:rows]
 
     def run_block(self, duration, variables=('membrane.V',),
                   log_interval=1.0, report_nan=True):

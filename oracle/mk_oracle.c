@@ -313,3 +313,137 @@ int oracle_run(OracleArgs* a)
     free(es.ev);
     return 0;
 }
+
+/* ------------------------------------------------- fiber-tissue run
+ * FiberTissueSimulation's sim_step (myokit/_sim/fiber_tissue.c:1003-1160):
+ * per step, the homogeneous stencil of each grid (diff_step) at t, the
+ * junction current of diff_step_fiber_tissue (openclsim.cl:601-628) added
+ * to the fiber cell and subtracted from the tissue cell, then the cell
+ * step of every fiber cell (paced where x < nx_paced: the reference's
+ * explicit cell list, fiber_tissue.py:195-198) and every tissue cell
+ * (unpaced: `Real pace = 0`, openclsim.cl:324-329). The two cell models
+ * come in as function pointers; states are AoS per part. Logs V of every
+ * cell of both parts at the log points (fiber cells first). */
+typedef void (*OracleCellFn)(double*, double, double, double, double,
+                             const double*, double*, int);
+
+typedef struct {
+    OracleCellFn cell_f, cell_t;
+    int nsf, nst, ivf, ivt;
+    long nfx, nfy, ntx, nty, ctx, cty, nx_paced;
+    double gfx, gfy, gtx, gty, gft;
+    double* state_f;
+    double* state_t;
+    double tmin, tmax, dt;
+    const double* events;
+    long n_events;
+    double log_interval;
+    double* log_v;          /* [max_rows][nfx*nfy + ntx*nty] */
+    double* log_time;
+    double* log_pace;
+    long max_rows;
+    /* outputs */
+    long rows;
+    double t_end;
+    long steps;
+} OracleFtArgs;
+
+/* diff_step (openclsim.cl:384-435) on an AoS grid */
+static void o_stencil(const double* s, int ns, int iv, long nx, long ny,
+                      double gx, double gy, double* idiff)
+{
+    long c;
+    for (c = 0; c < nx * ny; ++c) {
+        const long ix = c % nx, iy = c / nx;
+        const long of1 = c * ns + iv;
+        double i;
+        if (nx > 1) {
+            if (ix == 0) i = gx * (s[of1] - s[of1 + ns]);
+            else if (ix == nx - 1) i = gx * (s[of1] - s[of1 - ns]);
+            else i = gx * (2 * s[of1] - s[of1 - ns] - s[of1 + ns]);
+        } else {
+            i = 0;
+        }
+        if (ny > 1) {
+            const long up = of1 + ns * nx, dn = of1 - ns * nx;
+            if (iy == 0) i += gy * (s[of1] - s[up]);
+            else if (iy == ny - 1) i += gy * (s[of1] - s[dn]);
+            else i += gy * (2 * s[of1] - s[dn] - s[up]);
+        }
+        idiff[c] = i;
+    }
+}
+
+int oracle_ft_run(OracleFtArgs* a)
+{
+    const long nf = a->nfx * a->nfy, nt = a->ntx * a->nty;
+    double* idf = (double*)calloc(nf, sizeof(double));
+    double* idt = (double*)calloc(nt, sizeof(double));
+    OESys es;
+    double t = a->tmin, tnext_log = a->tmin, tnext_pace, pace;
+    unsigned long istep = 1, inext_log = 0;
+    long rows = 0, steps = 0, c, k;
+    int r;
+
+    r = o_esys_init(&es, a->tmin, a->events, a->n_events);
+    if (r) { free(idf); free(idt); free(es.ev); return r; }
+    tnext_pace = es.tnext;
+    pace = es.level;
+    for (;;) {
+        const int logging = t >= tnext_log;
+        int inter = 0;
+        double dt = a->tmin + (double)istep * a->dt - t, d;
+        d = a->tmax - t; if (d > 0 && d < dt) { dt = d; inter = 1; }
+        d = tnext_pace - t; if (d > 0 && d < dt) { dt = d; inter = 1; }
+        d = tnext_log - t; if (d > 0 && d < dt) { dt = d; inter = 1; }
+        if (!inter) istep++;
+
+        o_stencil(a->state_f, a->nsf, a->ivf, a->nfx, a->nfy, a->gfx, a->gfy,
+                  idf);
+        o_stencil(a->state_t, a->nst, a->ivt, a->ntx, a->nty, a->gtx, a->gty,
+                  idt);
+        for (k = 0; k < a->nfy; ++k) {
+            const long iff = (a->nfx - 1) + k * a->nfx;
+            const long ift = a->ctx + (a->cty + k) * a->ntx;
+            const double i = a->gft * (a->state_f[iff * a->nsf + a->ivf]
+                                       - a->state_t[ift * a->nst + a->ivt]);
+            idf[iff] += i;
+            idt[ift] -= i;
+        }
+        if (logging && rows < a->max_rows) {
+            double* row = a->log_v + rows * (nf + nt);
+            for (c = 0; c < nf; ++c) row[c] = a->state_f[c * a->nsf + a->ivf];
+            for (c = 0; c < nt; ++c)
+                row[nf + c] = a->state_t[c * a->nst + a->ivt];
+            a->log_time[rows] = t;
+            a->log_pace[rows] = pace;
+        }
+        for (c = 0; c < nf; ++c) {
+            const long ix = c % a->nfx;
+            a->cell_f(a->state_f + c * a->nsf, t, dt,
+                      ix < a->nx_paced ? pace : 0.0, idf[c], NULL, NULL, 0);
+        }
+        for (c = 0; c < nt; ++c)
+            a->cell_t(a->state_t + c * a->nst, t, dt, 0.0, idt[c], NULL, NULL,
+                      0);
+        if (logging) {
+            rows++;
+            inext_log++;
+            tnext_log = a->tmin + (double)inext_log * a->log_interval;
+        }
+        t += dt;
+        steps++;
+        r = o_advance(&es, t);
+        if (r) { free(idf); free(idt); free(es.ev); return r; }
+        tnext_pace = es.tnext;
+        pace = es.level;
+        if (t >= a->tmax) break;
+    }
+    a->rows = rows;
+    a->t_end = t;
+    a->steps = steps;
+    free(idf);
+    free(idt);
+    free(es.ev);
+    return 0;
+}
