@@ -32,8 +32,9 @@
 #ifndef MK_MATH_H
 #define MK_MATH_H
 
-#ifdef __HIPCC_RTC__
-  /* Device build through hiprtc: no system headers. */
+#if defined(__HIPCC_RTC__) || defined(MK_MATH_DEVICE)
+  /* Device build (hiprtc, or hipcc with MK_MATH_DEVICE defined, as in
+   * exported kernels): no system headers. */
   typedef unsigned long long mk_u64;
   #define MK_FN static __device__ __forceinline__
   #define MK_BITS(d) __builtin_bit_cast(mk_u64, (d))

@@ -139,3 +139,27 @@ def run_both(sc, duration, precision, var='membrane.V', log_interval=0.5):
     r = o.run(duration, log_vars=[var], log_interval=log_interval)
     O = {'time': r['time'], 'V': r[var], 'state': o.state, 'raw': r}
     return P, O
+
+
+# The exported cable program of the HIP exporter test: LR1991, double
+# precision, 40 cells, dt = 2^-7 ms (every step time exact in binary, so no
+# pacing boundary splits a step), g = 9.5, the first 5 cells paced by a 2 ms
+# stimulus, all states logged every 1 ms (128 steps) for 30 ms. Built on the
+# CPU by __graft_entry__.build(), run by the GPU test.
+EXPORT_DIR = __import__('os').path.join(
+    __import__('os').path.dirname(__file__), '_built', 'hip_cable')
+EXPORT_CFG = dict(precision=64, n_cells=40, dt=2.0 ** -7, g=9.5, n_paced=5,
+                  duration=30.0, log_interval=1.0)
+
+
+def build_exported_cable():
+    """Exports the cable with the 'hip' exporter and builds it with hipcc."""
+    import subprocess
+    from myokit_amd import formats
+    e = formats.exporter('hip', **EXPORT_CFG)
+    e.runnable(EXPORT_DIR, load('lr1991'), myokit.pacing.blocktrain(1000, 2))
+    r = subprocess.run(['bash', __import__('os').path.join(EXPORT_DIR,
+                                                           'build.sh')],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError('Exported cable failed to build:\n' + r.stderr)
