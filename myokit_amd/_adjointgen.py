@@ -387,6 +387,19 @@ __device__ __forceinline__ double mk_rcp64(double b)
 """
 
 
+_ISLAND_RCP = """
+/* 1/b for the double-precision islands: the hardware reciprocal and one
+ * Newton step (error squared; ample for 1e-11); for b = 0, inf or NaN the
+ * plain reciprocal. */
+__device__ __forceinline__ double mk_rcp_isl(double b)
+{
+    const double r0 = __builtin_amdgcn_rcp(b);
+    const double r = __builtin_fma(__builtin_fma(-b, r0, 1.0), r0, r0);
+    return __builtin_isfinite(r) ? r : r0;
+}
+"""
+
+
 _VJP_ONEM_EXPF = """
 /* 1 - exp(-a) in single precision, for the Rush-Larsen partial
  * d y' / d inf. Formed as 1 - E it loses log2(1/a) bits to cancellation when
@@ -426,8 +439,8 @@ def generate_vjp_function(cm, precision, compute=None, islands=True):
             return _VJP_EXP + _VJP_RCP64 + re.sub(r'\bmk_exp\(',
                                                 'mk_exp_vjp(', src)
         head = _VJP_LMUL + _VJP_ONEM_EXPF
-        if 'mk_rcp64' in src or 'mk_exp_isl' in src or '_f64(' in src:
-            head = _ISLAND_EXP + _VJP_RCP64 + _ISLAND_GUARDS + head
+        if 'mk_rcp_isl' in src or 'mk_exp_isl' in src or '_f64(' in src:
+            head = _ISLAND_EXP + _ISLAND_RCP + _ISLAND_GUARDS + head
         return head + src
     if not (precision != DOUBLE_PRECISION and compute == DOUBLE_PRECISION):
         raise ValueError('Only a double-precision VJP for a single-precision'
@@ -521,6 +534,7 @@ def _island(r, e, k, tab):
     body = re.sub(r'\bmk_gmul\(', 'mk_gmul_f64(', body)
     body = re.sub(r'\bmk_gdiv\(', 'mk_gdiv_f64(', body)
     body = re.sub(r'\bmk_exp\(', 'mk_exp_isl(', body)
+    body = re.sub(r'\bmk_rcp64\(', 'mk_rcp_isl(', body)
     used = sorted(set(re.findall(r'\bisl_\w+', body)))
     jac = {u[4:]: 'J' + str(k) + '_' + u[4:] for u in used}
     lines = [tab + 'Real ' + jac[u[4:]] + ';' for u in used]
@@ -609,11 +623,19 @@ def _vjp_source(cm, precision, islands=True):
                              + r.div('-dt', vt) + ');')
                 D = '((Real)1 - ' + E + ')'
             else:
+                # E as the forward kernel computes it (generate_cell_function:
+                # 2^((-dt log2(e)) X) with X = 1 / tau), A = dt X for 1 - E
+                trhs = tau.rhs()
+                if isinstance(trhs, ex.Divide) and isinstance(
+                        trhs[0], ex.Number) and trhs[0].eval() == 1:
+                    rate = w.ex(trhs[1])
+                else:
+                    rate = '__builtin_amdgcn_rcpf(' + vt + ')'
                 A, D = 'A_' + str(i), 'D_' + str(i)
-                lines.append(tab + 'const Real ' + A + ' = '
-                             + r.div('dt', vt) + ';')
-                lines.append(tab + 'const Real ' + E + ' = ' + expf + '(-'
-                             + A + ');')
+                lines.append(tab + 'const Real ' + A + ' = dt * ' + rate
+                             + ';')
+                lines.append(tab + 'const Real ' + E + ' = __builtin_amdgcn_'
+                             'exp2f((-dt * 0x1.715476p+0f) * ' + rate + ');')
                 lines.append(tab + 'const Real ' + D + ' = mk_onem_expf('
                              + A + ', ' + E + ');')
             # y' = inf - (inf - x) E: dy'/dx = E, dy'/dinf = 1 - E,

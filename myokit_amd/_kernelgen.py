@@ -462,16 +462,23 @@ def generate_cell_function(cm, precision, native_maths=False):
             # the reference (openclsim.cl:358-364)
             trhs = rl[s][1].rhs()
             if precision == DOUBLE_PRECISION:
-                arg = '-dt / ' + tau
-            elif isinstance(trhs, ex.Divide) and isinstance(
-                    trhs[0], ex.Number) and trhs[0].eval() == 1:
-                # tau = 1 / X (the form the RL conversion writes), so
-                # -dt / tau = -dt * X without two reciprocals
-                arg = '-dt * ' + w.ex(trhs[1])
+                E = expf + '(-dt / ' + tau + ')'
             else:
-                arg = '-dt * __builtin_amdgcn_rcpf(' + tau + ')'
+                if isinstance(trhs, ex.Divide) and isinstance(
+                        trhs[0], ex.Number) and trhs[0].eval() == 1:
+                    # tau = 1 / X (the form the RL conversion writes), so
+                    # -dt / tau = -dt * X without two reciprocals
+                    rate = w.ex(trhs[1])
+                else:
+                    rate = '__builtin_amdgcn_rcpf(' + tau + ')'
+                # exp(-dt X) = 2^((-dt log2(e)) X): the first factor is the
+                # same for every cell of a step, so one multiply and one
+                # v_exp_f32 per gate; within 3 ulp of exp (OpenCL's bound)
+                # for dt X <= 1, and below 1e-8 in absolute terms beyond
+                E = ('__builtin_amdgcn_exp2f((-dt * 0x1.715476p+0f) * '
+                     + rate + ')')
             lines.append(tab + 'y[' + str(i) + '] = ' + inf + ' - (' + inf
-                         + ' - ' + x + ') * ' + expf + '(' + arg + ');')
+                         + ' - ' + x + ') * ' + E + ';')
         else:
             lines.append(tab + 'y[' + str(i) + '] = ' + x + ' + dt * '
                          + _cname(s, 'd_') + ';')
