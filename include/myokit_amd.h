@@ -184,7 +184,10 @@ int mkh_sim_clean(void* sim);
  *   "time_kernels"  1: bracket every launch with HIP events
  *   "block_threads" threads per workgroup for the next runs
  *   "driver"        force a driver (MKH_DRIVER_*), -1 = automatic
- *   "adj_release"   1: free the adjoint's trajectory and work buffers */
+ *   "adj_release"   1: free the adjoint's trajectory and work buffers
+ *   "adj_sheet"     0: on a persistent sheet, take the adjoint's reverse
+ *                   sweep one launch per step instead of one persistent
+ *                   launch per segment (default 1) */
 int mkh_sim_set_option(void* sim, const char* key, long long value);
 
 /* Kernel timing since the last reset (time_kernels = 1): total milliseconds

@@ -520,6 +520,14 @@ __device__ __forceinline__ float mk_powif(float x, int n)
 '''
 
 
+def adjoint_sheet_fits(cm, precision):
+    """Whether the persistent reverse sheet (mk_adj_sheet: mu of a 64 x 32
+    tile and abar with its ring in LDS) fits twice into a CU's 160 KiB."""
+    w = 8 if precision == DOUBLE_PRECISION else 4
+    lds = len(cm.states) * 32 * 64 * w + 34 * 66 * w + 64 + 4
+    return 2 * lds <= 160 * 1024
+
+
 def adjoint_waves_default(cm, precision, adjoint_precision=None):
     """Waves per SIMD the reverse kernels are compiled for (1: no limit).
     The reverse step waits on its loads at the start of every cell; one
@@ -593,6 +601,11 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_BCELLS ' + str(bcells),
         '#define MK_SCPT ' + str(cpt),
         '#define MK_SWAVES ' + str(waves),
+        '#define MK_ACPT 8',
+        '#define MK_ADJ_SHEET ' + ('1' if adjoint and int(coupling) == 1
+                                   and adjoint_sheet_fits(cm, precision)
+                                   and adjoint_precision in (None, precision)
+                                   else '0'),
         '#define MK_ADJ_WAVES ' + str(max(1, int(
             adjoint_waves_default(cm, precision, adjoint_precision)
             if adjoint_waves is None else adjoint_waves))),

@@ -300,3 +300,271 @@ mk_adj_probe(const MkArgs a, const MkAdj j, const int s)
     j.trace[(long long)s * j.n_probe + q] =
         (double)j.yn[(long long)MK_IVM * a.npad + j.probes[q]];
 }
+
+#if MK_ADJ_SHEET
+/* ------------------------------------------------------------------------
+ * Persistent reverse sweep on a 2D stencil sheet: every reverse step of a
+ * segment [s0, s1) in one cooperative launch.
+ *
+ * Each workgroup owns a tile of 64 x MK_ATH cells (MK_ATH = 4 * MK_ACPT);
+ * thread (lx, w) handles cells (x0 + lx, y0 + w * MK_ACPT + k), k <
+ * MK_ACPT, one after the other. mu of the tile stays in LDS for the whole
+ * segment, abar of the tile and a one-cell ring around it too. The
+ * potentials V_s and V_{s+1} come from the trajectory, complete in global
+ * memory, so only abar crosses tiles: after each step a tile writes the
+ * abar of its four edges with agent-scope atomic stores and publishes its
+ * step count; before the next step it waits for its neighbours' counts and
+ * copies their edges into its ring (the exchange of mk_sheet).
+ *
+ * A step has two phases. Phase 1 computes, for every cell of the tile, the
+ * coupling transpose gx (abar_c - abar_x) + gy (...) from abar_{s+1} and
+ * the cell's terms of dL/dgx, dL/dgy. After a barrier, phase 2 assembles
+ * lam = mu_{s+1} + coupling + loss, applies the cell VJP at y_s and writes
+ * mu_s and abar_s in place. The barrier orders every read of abar_{s+1}
+ * before the first write of abar_s, so one abar buffer suffices.
+ *
+ * j.yn is the segment's trajectory (y_{s0 + k} at j.yn + k * n_state *
+ * npad); j.mu_out and j.ab_out carry mu and abar between segments. flags:
+ * bit 0 = this segment ends the run (mu, abar of step s1 are zero), bit 1 =
+ * it starts the run (add the conductance terms of step 0 at the end). Every
+ * wait is bounded (sh.spin_limit); on a timeout the tile sets sh.err and
+ * returns, and its neighbours then time out too.
+ * ---------------------------------------------------------------------- */
+#define MK_ATH (4 * MK_ACPT)
+
+/* Coupling transpose at the tile cell (r, lx) from abar in LDS (ring
+ * included) and V_{s+1} in global memory; adds the cell's conductance
+ * terms to cx, cy. Same arithmetic as mk_adj_couple on the grid. */
+__device__ __forceinline__ Real mk_adj_sheet_couple(
+    const MkArgs& a, const Real (*at)[66], const Real* __restrict__ V1,
+    int c, int r, int lx, int ix, int iy, int nx, int ny, double& cx,
+    double& cy)
+{
+    const Real ac = at[r + 1][lx + 1], v = V1[c];
+    Real ax = (Real)0, ay = (Real)0, xs = (Real)0, ys = (Real)0;
+    if (nx > 1) {
+        if (ix > 0) { ax += ac - at[r + 1][lx]; xs += v - V1[c - 1]; }
+        if (ix < nx - 1) { ax += ac - at[r + 1][lx + 2]; xs += v - V1[c + 1]; }
+    }
+    if (ny > 1) {
+        if (iy > 0) { ay += ac - at[r][lx + 1]; ys += v - V1[c - nx]; }
+        if (iy < ny - 1) { ay += ac - at[r + 2][lx + 1]; ys += v - V1[c + nx]; }
+    }
+    cx += (double)ac * (double)xs;
+    cy += (double)ac * (double)ys;
+    return a.gx * ax + a.gy * ay;
+}
+
+__device__ __forceinline__ Real* mk_adj_edge(const MkSheet& sh, int par,
+                                             int tile, int e)
+{
+    return sh.edges + ((long long)(par * sh.tiles_x * sh.tiles_y + tile) * 4
+                       + e) * 64;
+}
+
+/* Waits until the neighbours in nb[4] have published `target` steps.
+ * Returns false on a timeout. */
+__device__ __forceinline__ bool mk_adj_sheet_wait(const MkSheet& sh,
+                                                  const int nb[4], int target,
+                                                  int* flag)
+{
+    if (threadIdx.x < 4) {
+        const int nbt = nb[threadIdx.x];
+        if (nbt >= 0) {
+            int spins = 0;
+            while (__hip_atomic_load(sh.cnt + nbt, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > sh.spin_limit) {
+                    __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    *flag = 1;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return !*flag;
+}
+
+/* Waits until the neighbours in nb[4] have published `target` steps and
+ * copies their abar edges of parity `par` into the ring of at. Returns
+ * false on a timeout. */
+__device__ __forceinline__ bool mk_adj_sheet_halo(
+    const MkSheet& sh, const int nb[4], int target, int par, Real (*at)[66],
+    int* flag)
+{
+    if (!mk_adj_sheet_wait(sh, nb, target, flag)) return false;
+    const int t = threadIdx.x;
+    if (t < 64) {
+        if (nb[0] >= 0) at[0][t + 1] = mk_xload(mk_adj_edge(sh, par, nb[0], 1) + t);
+    } else if (t < 128) {
+        if (nb[1] >= 0)
+            at[MK_ATH + 1][t - 63] = mk_xload(mk_adj_edge(sh, par, nb[1], 0) + t - 64);
+    } else if (t < 128 + MK_ATH) {
+        if (nb[2] >= 0)
+            at[t - 127][0] = mk_xload(mk_adj_edge(sh, par, nb[2], 3) + t - 128);
+    } else if (t < 128 + 2 * MK_ATH) {
+        if (nb[3] >= 0)
+            at[t - 127 - MK_ATH][65] =
+                mk_xload(mk_adj_edge(sh, par, nb[3], 2) + t - 128 - MK_ATH);
+    }
+    __syncthreads();
+    return true;
+}
+
+extern "C" __global__ void __launch_bounds__(256, 2)
+mk_adj_sheet(const MkArgs a, const MkAdj j, const MkSheet sh, const int s0,
+             const int s1, const int flags)
+{
+    __shared__ Real mul[MK_NS][MK_ATH][64];
+    __shared__ Real at[MK_ATH + 2][66];
+    __shared__ double red[2][4];
+    __shared__ int wg_flag;
+    if (threadIdx.x == 0) wg_flag = 0;
+    const bool last_seg = flags & 1, first_seg = flags & 2;
+    const int tile = blockIdx.x;
+    const int tx = tile % sh.tiles_x, ty = tile / sh.tiles_x;
+    const int lx = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nx = (int)a.nx, ny = (int)a.ny;
+    const int ix = tx * 64 + lx, gy0 = ty * MK_ATH;
+    const long long np = a.npad;
+    const long long SN = (long long)MK_NS * np;
+    const int nb[4] = {ty > 0 ? tile - sh.tiles_x : -1,
+                       ty + 1 < sh.tiles_y ? tile + sh.tiles_x : -1,
+                       tx > 0 ? tile - 1 : -1,
+                       tx + 1 < sh.tiles_x ? tile + 1 : -1};
+    double cx = 0.0, cy = 0.0;
+
+    /* mu and abar of step s1, with abar's ring from global memory */
+    for (int k = 0; k < MK_ACPT; ++k) {
+        const int r = w * MK_ACPT + k, iy = gy0 + r;
+        const bool act = ix < nx && iy < ny;
+        const int c = act ? ix + iy * nx : 0;
+        for (int i = 0; i < MK_NS; ++i)
+            mul[i][r][lx] = (act && !last_seg) ? j.mu_out[i * np + c] : (Real)0;
+        at[r + 1][lx + 1] = (act && !last_seg) ? j.ab_out[c] : (Real)0;
+    }
+    {
+        const int t = threadIdx.x;
+        int hx = -1, hy = -1, lr = 0, lc = 0;
+        if (t < 66) {
+            hx = tx * 64 - 1 + t; hy = gy0 - 1; lr = 0; lc = t;
+        } else if (t < 132) {
+            hx = tx * 64 - 1 + (t - 66); hy = gy0 + MK_ATH; lr = MK_ATH + 1;
+            lc = t - 66;
+        } else if (t < 132 + MK_ATH) {
+            hx = tx * 64 - 1; hy = gy0 + (t - 132); lr = t - 132 + 1; lc = 0;
+        } else if (t < 132 + 2 * MK_ATH) {
+            hx = tx * 64 + 64; hy = gy0 + (t - 132 - MK_ATH);
+            lr = t - 132 - MK_ATH + 1; lc = 65;
+        }
+        if (t < 132 + 2 * MK_ATH)
+            at[lr][lc] = (!last_seg && hx >= 0 && hy >= 0 && hx < nx && hy < ny)
+                             ? j.ab_out[hx + hy * nx] : (Real)0;
+    }
+    __syncthreads();
+
+    const int nsteps = s1 - s0;
+    for (int i = 0; i < nsteps; ++i) {
+        const int s = s1 - 1 - i;
+        if (i > 0 && !mk_adj_sheet_halo(sh, nb, i, (i - 1) & 1, at, &wg_flag))
+            return;
+        const Real* __restrict__ ys = j.yn + (long long)(s - s0) * SN;
+        const Real* __restrict__ V0 = ys + MK_IVM * np;       /* V_s */
+        const Real* __restrict__ V1 = V0 + SN;                 /* V_{s+1} */
+        /* Phase 1: coupling transpose from abar_{s+1} */
+        Real lc[MK_ACPT];
+#pragma unroll
+        for (int k = 0; k < MK_ACPT; ++k) {
+            const int r = w * MK_ACPT + k, iy = gy0 + r;
+            lc[k] = (Real)0;
+            if (ix < nx && iy < ny)
+                lc[k] = mk_adj_sheet_couple(a, at, V1, ix + iy * nx, r, lx, ix,
+                                            iy, nx, ny, cx, cy);
+        }
+        __syncthreads();
+        /* Phase 2: lam, the cell VJP at y_s, mu_s and abar_s in place */
+        const MkStep p = a.steps[s];
+        const int par = i & 1;
+#pragma unroll 1
+        for (int k = 0; k < MK_ACPT; ++k) {
+            const int r = w * MK_ACPT + k, iy = gy0 + r;
+            if (!(ix < nx && iy < ny)) continue;
+            const int c = ix + iy * nx;
+            Real lam[MK_NS], y[MK_NS], mu[MK_NS], fv[MK_NFA], fb[MK_NFA], abo;
+#pragma unroll
+            for (int q = 0; q < MK_NS; ++q) {
+                lam[q] = mul[q][r][lx];
+                y[q] = ys[q * np + c];
+            }
+#pragma unroll
+            for (int f = 0; f < MK_NF; ++f) fv[f] = a.fields[f * np + c];
+            Real lv = mk_adj_loss(j, c, V1[c], s);
+            Real ck = (Real)0;
+#pragma unroll
+            for (int q = 0; q < MK_ACPT; ++q) if (q == k) ck = lc[q];
+            lv += ck;
+            lam[MK_IVM] += lv;
+            mk_adj_cell(a, p, y, fv, V0, c, c, c, ix, iy, nx, ny, lam, mu, abo,
+                        fb);
+#pragma unroll
+            for (int q = 0; q < MK_NS; ++q) mul[q][r][lx] = mu[q];
+            at[r + 1][lx + 1] = abo;
+            if (r == 0) mk_xstore(mk_adj_edge(sh, par, tile, 0) + lx, abo);
+            if (r == MK_ATH - 1) mk_xstore(mk_adj_edge(sh, par, tile, 1) + lx, abo);
+            if (lx == 0) mk_xstore(mk_adj_edge(sh, par, tile, 2) + r, abo);
+            if (lx == 63) mk_xstore(mk_adj_edge(sh, par, tile, 3) + r, abo);
+#pragma unroll
+            for (int f = 0; f < MK_NF; ++f) j.gfield[f * np + c] += (double)fb[f];
+        }
+        /* Edge stores complete, then the step count */
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(sh.cnt + tile, i + 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+    /* mu and abar of step s0 for the previous segment; at the start of the
+     * run, the conductance terms of step 0 instead */
+    if (first_seg) {
+        if (!mk_adj_sheet_halo(sh, nb, nsteps, (nsteps - 1) & 1, at, &wg_flag))
+            return;
+        const Real* __restrict__ V0 = j.yn + MK_IVM * np;
+        for (int k = 0; k < MK_ACPT; ++k) {
+            const int r = w * MK_ACPT + k, iy = gy0 + r;
+            if (ix < nx && iy < ny)
+                (void)mk_adj_sheet_couple(a, at, V0, ix + iy * nx, r, lx, ix,
+                                          iy, nx, ny, cx, cy);
+        }
+    } else {
+        /* A neighbour reads its launch-start ring from j.ab_out; it has
+         * done so once it has finished its first step (with one step per
+         * segment nothing else orders that read before these writes). */
+        if (nsteps == 1 && !mk_adj_sheet_wait(sh, nb, 1, &wg_flag)) return;
+        for (int k = 0; k < MK_ACPT; ++k) {
+            const int r = w * MK_ACPT + k, iy = gy0 + r;
+            if (!(ix < nx && iy < ny)) continue;
+            const int c = ix + iy * nx;
+            for (int q = 0; q < MK_NS; ++q) j.mu_out[q * np + c] = mul[q][r][lx];
+            j.ab_out[c] = at[r + 1][lx + 1];
+        }
+    }
+    /* Conductance partial sums: one per tile */
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        cx += __shfl_down(cx, o, 64);
+        cy += __shfl_down(cy, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = cx;
+        red[1][threadIdx.x >> 6] = cy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        j.gx_acc[tile] += red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        j.gy_acc[tile] += red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+#endif

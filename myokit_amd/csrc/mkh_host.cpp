@@ -62,7 +62,7 @@ struct Program {
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
     hipFunction_t f_sheet = nullptr, f_junction = nullptr;
     hipFunction_t f_adj_step = nullptr, f_adj_block = nullptr,
-                  f_adj_probe = nullptr;
+                  f_adj_probe = nullptr, f_adj_sheet = nullptr;
     std::vector<char> code;
 };
 
@@ -1055,6 +1055,8 @@ struct Sim : SimBase {
             choose_driver();
         } else if (k == "adj_release") {
             if (v) adj.release();
+        } else if (k == "adj_sheet") {
+            no_sheet_rev = v == 0;
         } else {
             return fail(MKH_E_ARG, "unknown option " + k);
         }
@@ -1092,14 +1094,19 @@ struct Sim : SimBase {
 
     /* Adjoint work buffers, kept between calls (sizes permitting) and
      * released by the "adj_release" option or with the simulation. */
+    /* "adj_sheet" option: 0 keeps the sheet's reverse sweep on the
+     * one-step kernel (mk_adj_step) */
+    bool no_sheet_rev = false;
+
     struct AdjBufs {
         DevBuf<Step> steps;
         DevBuf<R> ckpt, traj, work, work2, mu[2], ab[2];
         DevBuf<double> gfield, gx, gy, loss, target, trace;
-        DevBuf<int> probe_of;
+        DevBuf<int> probe_of, rcnt;
         DevBuf<long long> probes;
         void release()
         {
+            rcnt.release();
             steps.release();
             ckpt.release();
             traj.release();
@@ -1275,6 +1282,22 @@ struct Sim : SimBase {
                               driver == MKH_DRIVER_BLOCK ||
                               driver == MKH_DRIVER_CELLS;
         const bool block_rev = driver == MKH_DRIVER_BLOCK && !trace_out;
+        /* The persistent reverse sheet (64 x 32 tiles, all co-resident;
+         * the forward sheet's edge buffer holds at least as many tiles) */
+        const int rtx = sh_tx, rty = (int)((cfg.ny + 31) / 32);
+        bool sheet_rev = false;
+        if (driver == MKH_DRIVER_SHEET && !trace_out && prog->f_adj_sheet &&
+            sh_edges.p && (size_t)rtx * rty * 4 * 64 * 2 <= sh_edges.n &&
+            (size_t)rtx * rty <= nparts) {
+            int per_cu = 0;
+            MKH_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, prog->f_adj_sheet, 256, 0));
+            hipDeviceProp_t prop;
+            MKH_HIP(hipGetDeviceProperties(&prop, cfg.device));
+            sheet_rev = (long long)per_cu * prop.multiProcessorCount >=
+                        (long long)rtx * rty && !no_sheet_rev;
+        }
+        if (sheet_rev) MKH_HIP(adj.rcnt.alloc((size_t)rtx * rty));
         DevBuf<R>& work = adj.work;
         DevBuf<R>& work2 = adj.work2;
         if (use_pers) MKH_HIP(work.alloc(SN));
@@ -1439,6 +1462,27 @@ struct Sim : SimBase {
                 MKH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
                 recompute_ms += ms;
             }
+            if (sheet_rev) {
+                /* every reverse step of the segment in one cooperative
+                 * launch; mu in LDS, carried in muA and abar in abA */
+                MkAdjT<R> js = j;
+                js.yn = slot(0);
+                js.mu_out = muA.p;
+                js.ab_out = abA.p;
+                MkSheetT<R> shr = sheet_args();
+                shr.tiles_y = rty;
+                shr.cnt = adj.rcnt.p;
+                shr.g0 = 0;
+                int i0 = (int)s0, i1 = (int)s1;
+                int flags = (sg == nseg - 1 ? 1 : 0) | (sg == 0 ? 2 : 0);
+                MKH_HIP(hipMemsetAsync(adj.rcnt.p, 0,
+                                       adj.rcnt.n * sizeof(int), stream));
+                void* params[] = {&a, &js, &shr, &i0, &i1, &flags};
+                MKH_HIP(hipModuleLaunchCooperativeKernel(
+                    prog->f_adj_sheet, (unsigned)(rtx * rty), 1, 1, 256, 1,
+                    1, 0, stream, params));
+                continue;
+            }
             if (block_rev) {
                 /* every reverse step of the segment in one launch; mu in
                  * place in muA, abar carried in abA */
@@ -1461,7 +1505,7 @@ struct Sim : SimBase {
             }
         }
         /* closing pass: conductance terms of step 0 (y_0 is slot 0) */
-        if (!block_rev) {
+        if (!block_rev && !sheet_rev) {
             int e = reverse_step(nullptr, slot(0), 0, -1);
             if (e) return e;
         }
@@ -1762,6 +1806,13 @@ int mkh_program_build(const char* source, const char* options,
                                  "mk_adj_probe") != hipSuccess) {
             prog->f_adj_step = prog->f_adj_block = prog->f_adj_probe =
                 nullptr;
+            (void)hipGetLastError();
+        }
+        /* the persistent reverse sheet: stencil kernels whose tile fits */
+        if (!prog->f_adj_step ||
+            hipModuleGetFunction(&prog->f_adj_sheet, prog->module,
+                                 "mk_adj_sheet") != hipSuccess) {
+            prog->f_adj_sheet = nullptr;
             (void)hipGetLastError();
         }
     }

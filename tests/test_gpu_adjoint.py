@@ -304,3 +304,39 @@ def test_adjoint_block_layout_matches_grid_fp64(K):
         assert x1 == pytest.approx(x2, rel=1e-11)
         np.testing.assert_allclose(g1['ina.gNa'], g2['ina.gNa'], rtol=1e-11,
                                    atol=1e-12 * np.max(np.abs(g2['ina.gNa'])))
+
+
+@pytest.mark.parametrize('shape,K', [((128, 96), 10000), ((100, 70), 37),
+                                     ((100, 70), 299)])
+def test_adjoint_persistent_sheet_matches_step_kernel_fp32(shape, K):
+    # Single-precision sheets run the reverse sweep in one persistent launch
+    # per segment (mk_adj_sheet: mu in LDS, abar exchanged between tiles).
+    # Per cell it does the arithmetic of the one-launch-per-step kernel in
+    # the same order, so the loss and the field gradients are identical; the
+    # conductance gradients differ only in how partial sums are grouped.
+    # (100, 70) has partial tiles in x and y; K = 37 gives several segments,
+    # K = 299 a one-step segment (300 steps).
+    nx, ny = shape
+    out = []
+    for sheet in (1, 0):
+        m = myokit.load_model('example')
+        s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                    ncells=shape, precision=32, rl=True)
+        s.set_step_size(0.01)
+        s.set_conductance(9.5, 6.0)
+        s.set_paced_cells(3, ny, 0, 0)
+        s.set_field('ina.gNa', np.linspace(12, 20, nx * ny).reshape(ny, nx))
+        s.set_engine_option('adj_sheet', sheet)
+        adj = TissueAdjoint(s, [(10, 10), (nx // 2, ny // 2),
+                                (nx - 1, ny - 1)], checkpoint_every=K)
+        dur = 3.0
+        target = np.full((adj.n_steps(dur), 3), -40.0)
+        out.append(adj.loss_and_gradient(dur, target))
+        print('adj_sheet', sheet, 'driver', s.engine_driver(), 'loss',
+              out[-1][0], 'dgx', out[-1][2], 'dgy', out[-1][3], flush=True)
+        assert s.engine_driver() == 'sheet'
+    (l1, g1, x1, y1), (l2, g2, x2, y2) = out
+    assert l1 == l2
+    np.testing.assert_array_equal(g1['ina.gNa'], g2['ina.gNa'])
+    assert x1 == pytest.approx(x2, rel=1e-12)
+    assert y1 == pytest.approx(y2, rel=1e-12)
