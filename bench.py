@@ -25,6 +25,57 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4              # MI355X peak engine clock
+SIMDS = 256 * 4              # 256 CUs x 4 SIMDs
+
+# mk_sheet on this workload (FP32), from rocprofv3 counters of 200-step
+# dispatches (profiles/mk_sheet_lr1991_1024x1024_fp32.md):
+# - HBM traffic per cell-step: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors of
+#   a 256 MiB device copy), 319 + 349 MiB per 2.10e8 cell-steps;
+# - per wave-step of 4 cells: 825 VALU instructions at 4 cycles and 252
+#   transcendental ones (quarter rate) at 16.
+SHEET_TRAFFIC_B_PER_CELL_STEP = (319 + 349) * 2.0**20 / (200 * 2.0**20)
+SHEET_VALU_CYCLES_PER_WAVE_STEP = 825 * 4 + 252 * 16
+SHEET_CELLS_PER_LANE = 4
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Achievable device-to-device copy bandwidth in GB/s (bytes read plus
+    bytes written per second), timed with HIP events around ``reps``
+    hipMemcpy calls of ``nbytes``."""
+    import ctypes
+    hip = ctypes.CDLL('libamdhip64.so')
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    ms = ctypes.c_float()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError('%s failed (%d)' % (what, rc))
+    ok(hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)), 'hipMalloc')
+    try:
+        ok(hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)),
+           'hipMalloc')
+        try:
+            ok(hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)), 'hipMemset')
+            ok(hip.hipEventCreate(ctypes.byref(e0)), 'hipEventCreate')
+            ok(hip.hipEventCreate(ctypes.byref(e1)), 'hipEventCreate')
+            ok(hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3), 'hipMemcpy')
+            ok(hip.hipEventRecord(e0, None), 'hipEventRecord')
+            for _ in range(reps):
+                ok(hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None),
+                   'hipMemcpyAsync')
+            ok(hip.hipEventRecord(e1, None), 'hipEventRecord')
+            ok(hip.hipEventSynchronize(e1), 'hipEventSynchronize')
+            ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1),
+               'hipEventElapsedTime')
+            hip.hipEventDestroy(e0)
+            hip.hipEventDestroy(e1)
+        finally:
+            hip.hipFree(b)
+    finally:
+        hip.hipFree(a)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
 def workload(nx, ny, precision):
@@ -137,6 +188,7 @@ def main():
     cells_per_launch = tm['cell_steps'] / max(tm['launches'], 1)
     avg_ms = tm['kernel_ms'] / max(tm['launches'], 1)
     achieved = bytes_per_cell_step * cells_per_launch / (avg_ms * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
 
     line = {
         'metric': 'cell-steps/s (2D 1024² LR1991 fp32 fwd); '
@@ -169,8 +221,29 @@ def main():
             'kernel': 'mk_' + s.engine_driver(), 'avg_launch_ms': avg_ms,
             'bytes_per_cell_step': bytes_per_cell_step,
             'cells_per_launch': cells_per_launch,
+            'copy_gbs': copy_gbs,
+            'frac_of_copy': achieved / copy_gbs,
         },
     }
+    if (args.precision == 32 and s.engine_driver() == 'sheet'
+            and (args.nx, args.ny) == (1024, 1024)):
+        # Counter-derived figures of this kernel on this workload: HBM bytes
+        # per launch and the VALU issue bound beside the HBM one
+        steps_per_launch = cells_per_launch / n
+        waves = n / (64 * SHEET_CELLS_PER_LANE)
+        model_us = (waves / SIMDS * SHEET_VALU_CYCLES_PER_WAVE_STEP
+                    / (CLOCK_GHZ * 1e3))
+        meas_us = avg_ms * 1e3 / steps_per_launch
+        line['roofline']['traffic'] = (SHEET_TRAFFIC_B_PER_CELL_STEP
+                                       * cells_per_launch)
+        line['roofline']['valu'] = {
+            'bound_us_per_time_step': model_us,
+            'measured_us_per_time_step': meas_us,
+            'frac': model_us / meas_us,
+            'model': '825 VALU x 4 + 252 transcendental x 16 cycles per '
+                     'wave-step of 4 cells, 4 SIMDs x 256 CUs, 2.4 GHz '
+                     '(rocprofv3 SQ counters)',
+        }
     if args.adjoint_ms > 0:
         fwd_ms = el / args.steps * 1e3 * args.adjoint_ms / seg
         n_adj, res = adjoint_leg(s, args.adjoint_ms, [0, 200],

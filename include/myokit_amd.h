@@ -228,6 +228,16 @@ int mkh_host_free(void* ptr);
  * steps. No counterpart in the reference. */
 int mkh_adj_timing(void* sim, double* ms3);
 
+/* The reverse kernel the last mkh_adj_run used: MKH_ADJ_REV_STEP (one
+ * launch per time step over all cells), MKH_ADJ_REV_BLOCK (one launch per
+ * segment, a workgroup per domain) or MKH_ADJ_REV_SHEET (one cooperative
+ * launch per segment over the sheet's tiles), -1 before the first run; and
+ * the number of reverse launches it made. No counterpart in the reference. */
+#define MKH_ADJ_REV_STEP 0
+#define MKH_ADJ_REV_BLOCK 1
+#define MKH_ADJ_REV_SHEET 2
+int mkh_adj_reverse(void* sim, int* kind, long long* launches);
+
 /* The forward quantity of mkh_adj_run's loss: V at the probe cells after
  * each of the N time steps of tmin..tmax, into trace[s * n_probes + p]
  * (n_rows >= N). Leaves the state unchanged. No counterpart in the

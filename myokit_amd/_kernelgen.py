@@ -94,6 +94,23 @@ class HipExpressionWriter:
         self.constants = {}
         # sub-expression code -> C name of a local holding its value
         self.subst = {}
+        # exp() code -> (C name of a power B^n, factor exp(m)), single
+        # precision exponential families
+        self.exp_members = {}
+
+    def set_exp_members(self, members):
+        """Writes each exponential in ``members`` (code -> (C name of B^n,
+        exp(m)), from :func:`exp_families`) as that power times the factor,
+        folding the factor into a literal coefficient where the
+        exponential is multiplied or divided by one."""
+        self.exp_members = dict(members)
+
+    def _member(self, e, coef=1.0):
+        p, f = self.exp_members[e.code()]
+        f *= coef
+        if f == 1.0:
+            return p
+        return '(' + p + ' * ' + real_literal(f, self.precision) + ')'
 
     def fn(self, name):
         d, f, fn = _FUNCS[name]
@@ -143,6 +160,11 @@ class HipExpressionWriter:
         return self._infix(e, '-')
 
     def _ex_Multiply(self, e):
+        if self.exp_members:
+            for a, b in ((e[0], e[1]), (e[1], e[0])):
+                if isinstance(a, ex.Exp) and isinstance(b, ex.Number) \
+                        and a.code() in self.exp_members:
+                    return self._member(a, b.eval())
         return self._infix(e, '*')
 
     # Double precision divides with IEEE '/' (bit-exact with the oracle).
@@ -164,6 +186,9 @@ class HipExpressionWriter:
         # OpenCL allows for single-precision division); a literal divisor
         # becomes a multiplication by its reciprocal.
         a, b = e
+        if isinstance(b, ex.Number) and isinstance(a, ex.Exp) \
+                and a.code() in self.exp_members:
+            return self._member(a, 1.0 / b.eval())
         if isinstance(b, ex.Number):
             return ('(' + self.ex(a) + ' * '
                     + real_literal(1.0 / b.eval(), self.precision) + ')')
@@ -198,6 +223,8 @@ class HipExpressionWriter:
         return self._func('sqrt', e)
 
     def _ex_Exp(self, e):
+        if self.exp_members and e.code() in self.exp_members:
+            return self._member(e)
         if self.precision == SINGLE_PRECISION:
             aff = self.affine(e[0])
             if aff is not None:
@@ -429,6 +456,122 @@ def forward_sweep(cm, w, tab='    '):
     return lines
 
 
+def exp_families(cm, w, max_power=24, v_range=200.0):
+    """Single precision: groups the exponentials exp(k V + m) of the
+    membrane potential V whose rates are integer multiples k = n k0 (n = 1
+    .. max_power) of a common rate k0, such as exp(-0.01 V), exp(-0.02 V),
+    exp(-0.04 V) and exp(-0.1 (V + 47.13)) in LR1991. A family evaluates
+    B = exp(k0 V) once (one v_exp_f32) and each member as B^n exp(m), the
+    powers by multiplications: one transcendental per family instead of one
+    per member, at a rounding error of about n ulp (below that of the
+    argument k V + m itself when |k V + m| is large).
+
+    A member is admitted only where nothing can leave the single-precision
+    range for |V| <= ``v_range``: n |k0| v_range <= 60 and |m| <= 40.
+
+    Returns ``(lines, members)``: the statements computing B and its
+    powers (to follow the state loads) and a map from each member's
+    expression code to ``(C name of B^n, exp(m))``, for
+    :meth:`HipExpressionWriter.set_exp_members`."""
+    vm = cm.model.label('membrane_potential')
+    if vm is None or vm not in cm.states:
+        return [], {}
+    xname = w.names[ex.Name(vm)]
+    found = {}
+    # exponentials multiplied or divided by a literal, which can absorb
+    # the factor exp(m)
+    scalable = set()
+
+    def walk(e):
+        for x in e:
+            walk(x)
+        if isinstance(e, ex.Exp):
+            aff = w.affine(e[0])
+            if aff is not None and aff[0] == xname:
+                found.setdefault(e.code(), (aff[1], aff[2]))
+        elif isinstance(e, ex.Multiply):
+            for a, b in ((e[0], e[1]), (e[1], e[0])):
+                if isinstance(a, ex.Exp) and isinstance(b, ex.Number):
+                    scalable.add(a.code())
+        elif isinstance(e, ex.Divide):
+            if isinstance(e[0], ex.Exp) and isinstance(e[1], ex.Number):
+                scalable.add(e[0].code())
+    for var in cm.order:
+        walk(var.rhs())
+    found = {c: km for c, km in found.items() if abs(km[1]) <= 40}
+    remaining = set(k for k, m in found.values())
+    lines, subst = [], {}
+    nfam = 0
+
+    def multiple(k, k0):
+        r = k / k0
+        n = int(round(r))
+        if n < 1 or n > max_power or abs(r - n) > 1e-9 * n:
+            return None
+        if n * abs(k0) * v_range > 60:
+            return None
+        return n
+    while remaining:
+        best = None
+        for k0 in sorted(remaining, key=abs):
+            mem = {k: multiple(k, k0) for k in remaining}
+            mem = {k: n for k, n in mem.items() if n is not None}
+            count = sum(1 for k, m in found.values() if k in mem)
+            if best is None or count > best[0]:
+                best = (count, k0, mem)
+        count, k0, mem = best
+        remaining -= set(mem) | {k0}
+        if count < 2:
+            continue
+        base = 'xe' + str(nfam)
+        # powers by an addition chain: each new power is the product of two
+        # known ones where possible, else by the binary method
+        have = {1: base}
+        plines = []
+        for n in sorted(set(mem.values())):
+            if n in have:
+                continue
+            pair = None
+            for a in sorted(have, reverse=True):
+                if n - a in have:
+                    pair = (a, n - a)
+                    break
+            if pair is None:
+                steps = []
+                q = n
+                while q not in have:
+                    steps.append(q)
+                    q = q // 2 if q % 2 == 0 else q - 1
+                for q in reversed(steps):
+                    a = q // 2 if q % 2 == 0 else q - 1
+                    have[q] = base + '_' + str(q)
+                    plines.append('const Real ' + have[q] + ' = ' + have[a]
+                                  + ' * ' + have[q - a] + ';')
+                continue
+            have[n] = base + '_' + str(n)
+            plines.append('const Real ' + have[n] + ' = ' + have[pair[0]]
+                          + ' * ' + have[pair[1]] + ';')
+        # Issue cost: a member evaluated directly is one fma and one
+        # quarter-rate v_exp_f32 (4 + 16 cycles); in the family it is a
+        # share of the powers (4 each) and, if m != 0 and no literal factor
+        # absorbs exp(m), one multiply (4). Keep the family if it gains.
+        members = [(c, k, m) for c, (k, m) in found.items() if k in mem]
+        saved = 20 * (len(members) - 1)
+        added = 4 * len(plines) + 4 * sum(
+            1 for c, k, m in members if m != 0 and c not in scalable)
+        if added >= saved:
+            continue
+        nfam += 1
+        lines.append('const Real ' + base + ' = __builtin_amdgcn_exp2f('
+                     + xname + ' * '
+                     + real_literal(k0 * 1.4426950408889634, SINGLE_PRECISION)
+                     + ');')
+        lines.extend(plines)
+        for code, k, m in sorted(members):
+            subst[code] = (have[mem[k]], math.exp(m))
+    return lines, subst
+
+
 def generate_cell_function(cm, precision, native_maths=False):
     """Returns the HIP source of ``mk_cell<LOG>`` for a CellModel."""
     w = cell_writer(cm, precision, native_maths)
@@ -444,7 +587,13 @@ def generate_cell_function(cm, precision, native_maths=False):
     lines.append('{')
     lines.append(tab + '(void)t; (void)dt; (void)pace; (void)idiff;'
                  ' (void)fv; (void)q;')
-    lines.extend(forward_sweep(cm, w, tab))
+    fam = []
+    if precision == SINGLE_PRECISION:
+        fam, members = exp_families(cm, w)
+        w.set_exp_members(members)
+    sweep = forward_sweep(cm, w, tab)
+    ns = len(cm.states)
+    lines.extend(sweep[:ns] + [tab + x for x in fam] + sweep[ns:])
     rl = cm.rl_states
     if cm.quantities:
         lines.append(tab + 'if (LOG) {')
@@ -520,12 +669,24 @@ __device__ __forceinline__ float mk_powif(float x, int n)
 '''
 
 
-def adjoint_sheet_fits(cm, precision):
-    """Whether the persistent reverse sheet (mk_adj_sheet: mu of a 64 x 32
-    tile and abar with its ring in LDS) fits twice into a CU's 160 KiB."""
+# Cells per thread of the persistent reverse sheet, as MK_ACPT in
+# csrc/mk_args.h (which the host library reads too)
+ADJ_SHEET_CPT = 8
+
+
+def adjoint_sheet_groups(cm, precision, cpt=ADJ_SHEET_CPT):
+    """Workgroups per CU (at most 4) that the LDS of the persistent reverse
+    sheet leaves room for: mk_adj_sheet keeps mu of its 64 x 4 cpt tile and
+    abar with a one-cell ring in LDS. 0 if one does not fit into a CU's
+    160 KiB. With 8 cells per thread (64 x 32 tiles) an LR1991 tile takes
+    73 KiB in single precision: two per CU, 254 registers per lane and no
+    spills; 4 cells per thread would allow four per CU but spill at the
+    128 registers that leaves, which costs more than the occupancy
+    gains."""
     w = 8 if precision == DOUBLE_PRECISION else 4
-    lds = len(cm.states) * 32 * 64 * w + 34 * 66 * w + 64 + 4
-    return 2 * lds <= 160 * 1024
+    th = 4 * cpt
+    lds = len(cm.states) * th * 64 * w + (th + 2) * 66 * w + 64 + 4
+    return min(4, (160 * 1024) // lds)
 
 
 def adjoint_waves_default(cm, precision, adjoint_precision=None):
@@ -589,6 +750,7 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
     i_vm = vm.index() if (vm is not None and vm.is_state()) else 0
     bcells = 2048 if precision == DOUBLE_PRECISION else 4096
     real = 'double' if precision == DOUBLE_PRECISION else 'float'
+    asg = adjoint_sheet_groups(cm, precision)
     parts = [
         '// Generated by myokit_amd for model: '
         + str(cm.model.name()).replace('\n', ' '),
@@ -601,11 +763,11 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_BCELLS ' + str(bcells),
         '#define MK_SCPT ' + str(cpt),
         '#define MK_SWAVES ' + str(waves),
-        '#define MK_ACPT 8',
         '#define MK_ADJ_SHEET ' + ('1' if adjoint and int(coupling) == 1
-                                   and adjoint_sheet_fits(cm, precision)
+                                   and asg > 0
                                    and adjoint_precision in (None, precision)
                                    else '0'),
+        '#define MK_ADJ_SWAVES ' + str(max(1, asg)),
         '#define MK_ADJ_WAVES ' + str(max(1, int(
             adjoint_waves_default(cm, precision, adjoint_precision)
             if adjoint_waves is None else adjoint_waves))),

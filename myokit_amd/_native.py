@@ -83,6 +83,7 @@ SIGNATURES = {
     'mkh_adj_run': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll, _i,
                          _pd, _pd, _pd]),
     'mkh_adj_timing': (_i, [_vp, _pd]),
+    'mkh_adj_reverse': (_i, [_vp, _pi, _pll]),
     'mkh_sim_copy_state': (_i, [_vp, _vp]),
     'mkh_ft_create': (_i, [_vp, _vp, _ll, _ll, _d,
                            ctypes.POINTER(ctypes.c_void_p)]),

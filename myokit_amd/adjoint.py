@@ -129,6 +129,19 @@ class TissueAdjoint:
             eng.h, _native.ptr(ms, ctypes.c_double)), 'mkh_adj_timing')
         return {'forward': ms[0], 'recompute': ms[1], 'reverse': ms[2]}
 
+    def reverse_kernel(self):
+        """The reverse kernel of the last ``loss_and_gradient`` and its
+        number of launches: ``('step' | 'block' | 'sheet', launches)``, or
+        ``(None, 0)`` before the first call."""
+        eng = getattr(self, '_eng', None)
+        if eng is None:
+            return None, 0
+        kind, n = ctypes.c_int(-1), ctypes.c_longlong(0)
+        _native.check(_native.lib().mkh_adj_reverse(
+            eng.h, ctypes.byref(kind), ctypes.byref(n)), 'mkh_adj_reverse')
+        names = {0: 'step', 1: 'block', 2: 'sheet'}
+        return names.get(kind.value), n.value
+
     def release(self):
         """Frees the device trajectory and work buffers kept between
         calls."""

@@ -308,7 +308,8 @@ mk_adj_probe(const MkArgs a, const MkAdj j, const int s)
  *
  * Each workgroup owns a tile of 64 x MK_ATH cells (MK_ATH = 4 * MK_ACPT);
  * thread (lx, w) handles cells (x0 + lx, y0 + w * MK_ACPT + k), k <
- * MK_ACPT, one after the other. mu of the tile stays in LDS for the whole
+ * MK_ACPT, one after the other. MK_ADJ_SWAVES workgroups per CU fit into
+ * LDS, and the registers are budgeted for as many waves per SIMD. mu of the tile stays in LDS for the whole
  * segment, abar of the tile and a one-cell ring around it too. The
  * potentials V_s and V_{s+1} come from the trajectory, complete in global
  * memory, so only abar crosses tiles: after each step a tile writes the
@@ -414,7 +415,7 @@ __device__ __forceinline__ bool mk_adj_sheet_halo(
     return true;
 }
 
-extern "C" __global__ void __launch_bounds__(256, 2)
+extern "C" __global__ void __launch_bounds__(256, MK_ADJ_SWAVES)
 mk_adj_sheet(const MkArgs a, const MkAdj j, const MkSheet sh, const int s0,
              const int s1, const int flags)
 {

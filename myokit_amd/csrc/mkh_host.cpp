@@ -220,6 +220,7 @@ struct SimBase {
                         const long long*, long long, const double*, long long,
                         int, double*, double*, double*, double*) = 0;
     virtual int adj_timing(double*) = 0;
+    virtual int adj_reverse(int*, long long*) = 0;
     virtual int copy_state_from(SimBase* src) = 0;
 };
 
@@ -1130,6 +1131,9 @@ struct Sim : SimBase {
     /* Wall time of the last adj_run's phases (ms): forward with
      * checkpoints, recomputation of earlier segments, reverse steps. */
     double adj_ms[3] = {0, 0, 0};
+    /* reverse kernel of the last adj_run (MKH_ADJ_REV_*) and its launches */
+    int adj_rev_kind = -1;
+    long long adj_rev_launches = 0;
 
     /* Discrete adjoint of a run from the current state (which it leaves
      * unchanged). The forward pass keeps a checkpoint at the start of every
@@ -1282,9 +1286,11 @@ struct Sim : SimBase {
                               driver == MKH_DRIVER_BLOCK ||
                               driver == MKH_DRIVER_CELLS;
         const bool block_rev = driver == MKH_DRIVER_BLOCK && !trace_out;
-        /* The persistent reverse sheet (64 x 32 tiles, all co-resident;
-         * the forward sheet's edge buffer holds at least as many tiles) */
-        const int rtx = sh_tx, rty = (int)((cfg.ny + 31) / 32);
+        /* The persistent reverse sheet: 64 x 4 MK_ACPT tiles, all
+         * co-resident; they are no more than the forward sheet's (4 *
+         * sheet_cpt rows), whose edge buffer they use */
+        const int rtx = sh_tx;
+        const int rty = (int)((cfg.ny + 4 * MK_ACPT - 1) / (4 * MK_ACPT));
         bool sheet_rev = false;
         if (driver == MKH_DRIVER_SHEET && !trace_out && prog->f_adj_sheet &&
             sh_edges.p && (size_t)rtx * rty * 4 * 64 * 2 <= sh_edges.n &&
@@ -1298,6 +1304,9 @@ struct Sim : SimBase {
                         (long long)rtx * rty && !no_sheet_rev;
         }
         if (sheet_rev) MKH_HIP(adj.rcnt.alloc((size_t)rtx * rty));
+        adj_rev_kind = sheet_rev ? MKH_ADJ_REV_SHEET
+                       : block_rev ? MKH_ADJ_REV_BLOCK : MKH_ADJ_REV_STEP;
+        adj_rev_launches = 0;
         DevBuf<R>& work = adj.work;
         DevBuf<R>& work2 = adj.work2;
         if (use_pers) MKH_HIP(work.alloc(SN));
@@ -1429,6 +1438,7 @@ struct Sim : SimBase {
             void* params[] = {&a, &js, &sii};
             MKH_HIP(hipModuleLaunchKernel(prog->f_adj_step, cgrid, 1, 1, bt,
                                           1, 1, 0, stream, params, nullptr));
+            adj_rev_launches++;
             mu_in = mu_out;
             ab_in = ab_out;
             mu_out = (mu_out == muA.p) ? muB.p : muA.p;
@@ -1481,6 +1491,7 @@ struct Sim : SimBase {
                 MKH_HIP(hipModuleLaunchCooperativeKernel(
                     prog->f_adj_sheet, (unsigned)(rtx * rty), 1, 1, 256, 1,
                     1, 0, stream, params));
+                adj_rev_launches++;
                 continue;
             }
             if (block_rev) {
@@ -1496,6 +1507,7 @@ struct Sim : SimBase {
                 MKH_HIP(hipModuleLaunchKernel(prog->f_adj_block,
                                               (unsigned)ndom, 1, 1, bt, 1, 1,
                                               0, stream, params, nullptr));
+                adj_rev_launches++;
                 continue;
             }
             for (long long si = s1 - 1; si >= s0; --si) {
@@ -1558,6 +1570,13 @@ struct Sim : SimBase {
     int adj_timing(double* ms) override
     {
         for (int k = 0; k < 3; ++k) ms[k] = adj_ms[k];
+        return 0;
+    }
+
+    int adj_reverse(int* kind, long long* launches) override
+    {
+        *kind = adj_rev_kind;
+        *launches = adj_rev_launches;
         return 0;
     }
 
@@ -2077,6 +2096,13 @@ int mkh_adj_timing(void* sim, double* ms3)
     CHECK_SIM();
     if (!ms3) return fail(MKH_E_ARG, "null argument");
     return SIM->adj_timing(ms3);
+}
+
+int mkh_adj_reverse(void* sim, int* kind, long long* launches)
+{
+    CHECK_SIM();
+    if (!kind || !launches) return fail(MKH_E_ARG, "null argument");
+    return SIM->adj_reverse(kind, launches);
 }
 
 int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,

@@ -170,6 +170,9 @@ class SimulationOpenCL:
         self._dev_newer = False
         self._engine = None
         self._engines = OrderedDict()   # kernel key -> engine (recent last)
+        # generator inputs -> (kernel source, CellModel): the source of a
+        # repeated run is not generated again
+        self._sources = {}
         if not model.is_valid():
             model.validate()
         if model.has_interdependent_components():
@@ -711,17 +714,27 @@ class SimulationOpenCL:
 
     def _get_engine(self, quantities, adjoint=False, adjoint_precision=None):
         fields = list(self._fields.keys())
-        cm = _kernelgen.CellModel(self._model, self._bound_variables,
-                                  self._rl_states, fields, quantities)
         coupling = self._coupling()
         tune = self._tuning
-        src = _kernelgen.generate_kernel(
-            cm, self._precision, coupling, self._block_threads,
-            self._native_math, adjoint=adjoint,
-            adjoint_precision=adjoint_precision,
-            tile_cpt=tune.get('sheet_cpt'),
-            tile_waves=tune.get('sheet_waves'),
-            adjoint_waves=tune.get('adjoint_waves'))
+        gkey = (tuple(fields), tuple(quantities), int(coupling),
+                self._block_threads, self._native_math, bool(adjoint),
+                adjoint_precision, tuple(sorted(tune.items())))
+        hit = self._sources.get(gkey)
+        if hit is None:
+            cm = _kernelgen.CellModel(self._model, self._bound_variables,
+                                      self._rl_states, fields, quantities)
+            src = _kernelgen.generate_kernel(
+                cm, self._precision, coupling, self._block_threads,
+                self._native_math, adjoint=adjoint,
+                adjoint_precision=adjoint_precision,
+                tile_cpt=tune.get('sheet_cpt'),
+                tile_waves=tune.get('sheet_waves'),
+                adjoint_waves=tune.get('adjoint_waves'))
+            if len(self._sources) >= 16:
+                self._sources.pop(next(iter(self._sources)))
+            self._sources[gkey] = (src, cm)
+        else:
+            src, cm = hit
         opts = _kernelgen.compile_options(self._precision)
         key = (src, opts, self._ntotal, self._dims)
         if self._engine is not None and self._engine_key == key:

@@ -388,3 +388,63 @@ void run(const double* x, double* y, long n)
     assert np.max(np.abs(y[ok] - ref[ok]) / ref[ok]) < 1e-11
     assert y[-5] == 1.0 and y[-4] == 0.0 and np.isinf(y[-3])
     assert np.isinf(y[-2]) and y[-1] == 0.0
+
+
+def test_inline_legacy_multiply_waits_for_its_operands(tmp_path):
+    # mk_lmul is inline assembly, before which the compiler inserts no wait
+    # states. On gfx950 a VALU read of a transcendental's result (v_rcp_f32,
+    # v_exp_f32) needs one; without it the multiply reads the register's
+    # previous value, deterministically for a given schedule. The reverse
+    # kernels of a single-precision adjoint are compiled here (hiprtc needs
+    # no GPU) and every v_mul_legacy_f32 must follow an s_nop.
+    import re
+    from myokit_amd import _native
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, None, ncells=(64, 32), precision=32,
+                                rl=True)
+    s.set_field('ina.gNa', np.ones((32, 64)))
+    cm = _kernelgen.CellModel(s._model, s._bound_variables, s._rl_states,
+                              [s._model.get('ina.gNa')],
+                              [s._model.get('membrane.V')])
+    src = _kernelgen.generate_kernel(cm, 32, 1, adjoint=True)
+    opts = _kernelgen.compile_options(32) + ' --save-temps'
+    prog = ctypes.c_void_p()
+    log = ctypes.create_string_buffer(1 << 16)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        _native.check(_native.lib().mkh_program_build(
+            src.encode(), opts.encode(), None, -1, ctypes.byref(prog), log,
+            1 << 16), 'mkh_program_build')
+    finally:
+        os.chdir(cwd)
+    _native.lib().mkh_program_free(prog)
+    asm = [f for f in os.listdir(tmp_path) if f.endswith('.s')]
+    assert asm
+    text = open(os.path.join(tmp_path, asm[0])).read()
+    lines = [x.strip() for x in text.splitlines()
+             if x.strip() and not x.strip().startswith((';', '.'))]
+    n = 0
+    for k, x in enumerate(lines):
+        if x.startswith('v_mul_legacy_f32'):
+            n += 1
+            assert lines[k - 1].startswith('s_nop'), lines[k - 2:k + 1]
+    assert n >= 3 * 50      # three reverse kernels, dozens of guards each
+
+
+def test_reverse_sheet_tile_height_matches_header():
+    # The generator sizes the reverse sheet's LDS with ADJ_SHEET_CPT; the
+    # kernel and the host library take MK_ACPT from csrc/mk_args.h.
+    import re
+    text = open(os.path.join(CSRC, 'mk_args.h')).read()
+    m = re.search(r'#define MK_ACPT (\d+)', text)
+    assert m and int(m.group(1)) == _kernelgen.ADJ_SHEET_CPT
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, None, ncells=(64, 32), precision=32,
+                                rl=True)
+    cm = _kernelgen.CellModel(s._model, s._bound_variables, s._rl_states,
+                              [], [s._model.get('membrane.V')])
+    assert _kernelgen.adjoint_sheet_groups(cm, 32) == 2
+    src = _kernelgen.generate_kernel(cm, 32, 1, adjoint=True)
+    assert '#define MK_ADJ_SHEET 1' in src
+    assert '#define MK_ADJ_SWAVES 2' in src
