@@ -173,6 +173,8 @@ class SimulationOpenCL:
         # generator inputs -> (kernel source, CellModel): the source of a
         # repeated run is not generated again
         self._sources = {}
+        # number of quantities -> column map of run_block (every cell)
+        self._block_colmaps = {}
         if not model.is_valid():
             model.validate()
         if model.has_interdependent_components():
@@ -1057,8 +1059,12 @@ class SimulationOpenCL:
                 raise ValueError('Cannot log <' + var.qname() + '> per cell.')
             quantities.append(var)
         n = self._ntotal
-        colmap = np.arange(len(quantities) * n, dtype=np.int32).reshape(
-            len(quantities), n)
+        colmap = self._block_colmaps.get(len(quantities))
+        if colmap is None:
+            colmap = np.arange(len(quantities) * n, dtype=np.int32).reshape(
+                len(quantities), n)
+            colmap.setflags(write=False)
+            self._block_colmaps[len(quantities)] = colmap
         tmin = self._time
         values, times, paces, err = self._execute(
             tmin, tmin + duration, quantities, colmap, colmap.size,
