@@ -31,11 +31,13 @@ SIMDS = 256 * 4              # 256 CUs x 4 SIMDs
 # mk_sheet on this workload (FP32), from rocprofv3 counters of 200-step
 # dispatches (profiles/mk_sheet_lr1991_1024x1024_fp32.md):
 # - HBM traffic per cell-step: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors of
-#   a 256 MiB device copy), 319 + 349 MiB per 2.10e8 cell-steps;
-# - per wave-step of 4 cells: 825 VALU instructions at 4 cycles and 252
+#   a 256 MiB device copy), 307.6 + 340.8 MiB per 2.10e8 cell-steps;
+# - per wave-step of 4 cells: 840 VALU instructions at 4 cycles and 224
 #   transcendental ones (quarter rate) at 16.
-SHEET_TRAFFIC_B_PER_CELL_STEP = (319 + 349) * 2.0**20 / (200 * 2.0**20)
-SHEET_VALU_CYCLES_PER_WAVE_STEP = 825 * 4 + 252 * 16
+SHEET_TRAFFIC_B_PER_CELL_STEP = (307.6 + 340.8) * 2.0**20 / (200 * 2.0**20)
+SHEET_VALU = 840
+SHEET_TRANS = 224
+SHEET_VALU_CYCLES_PER_WAVE_STEP = SHEET_VALU * 4 + SHEET_TRANS * 16
 SHEET_CELLS_PER_LANE = 4
 
 
@@ -240,9 +242,9 @@ def main():
             'bound_us_per_time_step': model_us,
             'measured_us_per_time_step': meas_us,
             'frac': model_us / meas_us,
-            'model': '825 VALU x 4 + 252 transcendental x 16 cycles per '
+            'model': '%d VALU x 4 + %d transcendental x 16 cycles per '
                      'wave-step of 4 cells, 4 SIMDs x 256 CUs, 2.4 GHz '
-                     '(rocprofv3 SQ counters)',
+                     '(rocprofv3 SQ counters)' % (SHEET_VALU, SHEET_TRANS),
         }
     if args.adjoint_ms > 0:
         fwd_ms = el / args.steps * 1e3 * args.adjoint_ms / seg
