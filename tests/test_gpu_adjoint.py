@@ -240,7 +240,10 @@ def test_adjoint_sheet_driver_matches_grid_fp64(K):
     # 128 x 64 cells: large enough for the persistent sheet driver, whose
     # forward and recomputation passes store the trajectory from registers.
     # In double precision the sheet is bit-exact with the grid driver, so
-    # the gradients must be identical. K = 70 exercises recomputation.
+    # the loss and the field gradients are identical; the reverse pass runs
+    # on the persistent reverse sheet (one workgroup per CU in double
+    # precision), whose conductance partial sums are grouped by tile, so
+    # dL/dgx and dL/dgy agree to 1e-12. K = 70 exercises recomputation.
     out = []
     for force_grid in (False, True):
         m = myokit.load_model('example')
@@ -249,6 +252,7 @@ def test_adjoint_sheet_driver_matches_grid_fp64(K):
         s.set_step_size(0.01)
         s.set_conductance(9.5, 6.0)
         s.set_paced_cells(3, 64, 0, 0)
+        s.set_field('ina.gNa', np.linspace(14, 18, 128 * 64).reshape(64, 128))
         if force_grid:
             s.set_engine_option('driver', 2)
         adj = TissueAdjoint(s, [(10, 10), (60, 30), (127, 63)],
@@ -256,11 +260,16 @@ def test_adjoint_sheet_driver_matches_grid_fp64(K):
         dur = 3.0
         target = np.full((adj.n_steps(dur), 3), -40.0)
         out.append(adj.loss_and_gradient(dur, target))
-        print('driver', s.engine_driver(), 'loss', out[-1][0], 'dgx',
-              out[-1][2], 'dgy', out[-1][3], flush=True)
+        print('driver', s.engine_driver(), 'reverse', adj.reverse_kernel(),
+              'loss', out[-1][0], 'dgx', out[-1][2], 'dgy', out[-1][3],
+              flush=True)
         assert s.engine_driver() == ('grid' if force_grid else 'sheet')
-    (l1, _, x1, y1), (l2, _, x2, y2) = out
-    assert l1 == l2 and x1 == x2 and y1 == y2
+        assert adj.reverse_kernel()[0] == ('step' if force_grid else 'sheet')
+    (l1, g1, x1, y1), (l2, g2, x2, y2) = out
+    assert l1 == l2
+    np.testing.assert_array_equal(g1['ina.gNa'], g2['ina.gNa'])
+    assert x1 == pytest.approx(x2, rel=1e-12)
+    assert y1 == pytest.approx(y2, rel=1e-12)
 
 
 @pytest.mark.parametrize('K', [10000, 37])
