@@ -115,6 +115,286 @@ def cpu_baseline(nx, ny, sample_ms, threads):
     return r['steps'] * nx * ny / el, r['steps'], el
 
 
+# ---------------------------------------------------------------------------
+# The other configurations of the scope table, each as its own workload
+# (``--workload``): a bounded window of the configuration, timed like the
+# sheet, with the algorithmic bytes per cell-step of its dominant kernel.
+# ---------------------------------------------------------------------------
+
+def _scales(rng, n):
+    return np.exp(rng.uniform(np.log(0.5), np.log(2.0), n))
+
+
+def _config1(precision=64):
+    """200-cell LR1991 cable, FE, dt 0.005 ms, gx 9.5, cells 0-4 paced at
+    1 Hz (2 ms, offset 0), V of every cell logged every 1 ms."""
+    import myokit_amd as myokit
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=200, precision=precision)
+    s.set_step_size(0.005)
+    s.set_conductance(9.5)
+    s.set_paced_cells(5)
+    return s, 1.0
+
+
+def _config3_cells(precision=32, n=65536, seed=3):
+    """65,536 uncoupled LR1991 cells (diffusion off, all paced at 1 Hz),
+    gNa, gCa (gsi) and gK scaled log-uniform in [0.5, 2] from a seed, FE,
+    dt 0.005 ms, V logged every 0.5 ms."""
+    import myokit_amd as myokit
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=n, diffusion=False,
+                                precision=precision)
+    s.set_step_size(0.005)
+    rng = np.random.default_rng(seed)
+    for name in ('ina.gNa', 'ica.gCa', 'ik.IK.gK'):
+        s.set_field(name, m.get(name).rhs().eval() * _scales(rng, n))
+    return s, 0.5
+
+
+def _config3_cables(precision=32, ncables=256, seed=11):
+    """256 LR1991 cables with lengths drawn uniformly in [50, 2000] from a
+    seed, chains of g = 9.5 built with set_connections, the first 5 cells
+    of each paced at 1 Hz, FE, dt 0.005 ms, V logged every 0.5 ms; all
+    cables in one launch."""
+    import myokit_amd as myokit
+    m = myokit.load_model('example')
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(50, 2001, ncables)
+    n = int(lengths.sum())
+    rows, paced, off = [], [], 0
+    for L in lengths:
+        i = np.arange(off, off + L - 1)
+        rows.append(np.stack([i, i + 1, np.full(L - 1, 9.5)], axis=1))
+        paced += list(range(off, off + 5))
+        off += L
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=n, precision=precision)
+    s.set_step_size(0.005)
+    s.set_connections(np.concatenate(rows))
+    s.set_paced_cell_list(paced)
+    return s, 0.5
+
+
+def _smooth_field(rng, n, lo=0.7, hi=1.3):
+    x = np.arange(n) / n
+    f = np.zeros(n)
+    for _ in range(8):
+        f += rng.uniform(0.2, 1.0) * np.sin(
+            2 * np.pi * rng.uniform(0.5, 6.0) * x + rng.uniform(0, 2 * np.pi))
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+def _config4(seed=7, gx=9.5):
+    """1000-cell LR1991 cable, FP64, FE, dt 0.005 ms, cells 0-4 paced, gNa
+    and gCa scale fields smooth in [0.7, 1.3] (8 seeded sinusoids)."""
+    import myokit_amd as myokit
+    m = myokit.load_model('example')
+    rng = np.random.default_rng(seed)
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=1000, precision=64)
+    s.set_step_size(0.005)
+    s.set_conductance(gx)
+    s.set_paced_cells(5)
+    s.set_field('ina.gNa', 16.0 * _smooth_field(rng, 1000))
+    s.set_field('ica.gCa', 0.09 * _smooth_field(rng, 1000))
+    return s
+
+
+def _config5(sets=32, n=262144, seed=21):
+    """``sets`` parameter sets of a seeded random geometric graph of ``n``
+    LR1991 cells in the unit square (mean degree 6, g uniform in [5, 12]),
+    run as one batch (the disjoint union of one copy per set), FP32, RL,
+    dt 0.01 ms, gNa, gCa and gK scaled log-uniform in [0.5, 2] per cell and
+    set; cells within 0.05 of the square's left edge paced."""
+    import myokit_amd as myokit
+    from scipy.spatial import cKDTree
+    m = myokit.load_model('example')
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0, 1, (n, 2))
+    r = np.sqrt(6.0 / (np.pi * n))
+    pairs = cKDTree(pts).query_pairs(r, output_type='ndarray')
+    g = rng.uniform(5, 12, len(pairs))
+    base = np.stack([pairs[:, 0], pairs[:, 1], g], axis=1).astype(float)
+    rows = np.concatenate([base + np.array([k * n, k * n, 0.0])
+                           for k in range(sets)])
+    left = np.nonzero(pts[:, 0] < 0.05)[0]
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=n * sets, precision=32, rl=True)
+    s.set_step_size(0.01)
+    s.set_connections(rows)
+    s.set_paced_cell_list(np.concatenate([left + k * n
+                                          for k in range(sets)]).tolist())
+    for name in ('ina.gNa', 'ica.gCa', 'ik.IK.gK'):
+        s.set_field(name, m.get(name).rhs().eval()
+                    * _scales(rng, n * sets))
+    return s, len(base)
+
+
+def _time_blocks(s, seg, log_interval, steps, warmup):
+    """Wall time of ``steps`` run_block calls of ``seg`` ms after
+    ``warmup``, and the HIP-event kernel timing of one more."""
+    for _ in range(warmup):
+        s.run_block(seg, ['membrane.V'], log_interval=log_interval)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = s.run_block(seg, ['membrane.V'], log_interval=log_interval)
+    el = time.perf_counter() - t0
+    assert np.all(np.isfinite(out['membrane.V']))
+    s.set_engine_option('time_kernels', 1)
+    s.engine_timing(reset=True)
+    s.run_block(seg, ['membrane.V'], log_interval=log_interval)
+    tm = s.engine_timing(reset=True)
+    s.set_engine_option('time_kernels', 0)
+    return el, tm
+
+
+def _cpu_sample(o, sample_ms, threads):
+    o.threads = threads
+    o.run(o.dt, log_vars=['membrane.V'], log_interval=1e9, cells=[0])
+    t0 = time.perf_counter()
+    r = o.run(sample_ms, log_vars=['membrane.V'], log_interval=1e9,
+              cells=[0])
+    el = time.perf_counter() - t0
+    return r['steps'] * o.n / el, r['steps'], el
+
+
+def other_workload(args):
+    """One JSON line for a configuration other than the 1024 x 1024
+    sheet."""
+    import myokit_amd as myokit
+    from oracle.cgen import OracleSimulation
+    w = args.workload
+    threads = min(16, len(os.sched_getaffinity(0)))
+    line = {'metric': 'cell-steps/s', 'unit': 'cell-steps/s', 'n_gpus': 1,
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'data': 'synthetic', 'steps': args.steps, 'warmup': args.warmup}
+    oracle = None
+    if w in ('cable', 'cells', 'cables'):
+        prec = 64 if w == 'cable' else args.precision
+        wb = 4 if prec == 32 else 8
+        if w == 'cable':
+            s, li = _config1(prec)
+            seg = 100.0
+            bpcs = 2 * 8 * wb
+            desc = _config1.__doc__
+            oracle = OracleSimulation(myokit.load_model('example'),
+                                      myokit.pacing.blocktrain(1000, 2),
+                                      ncells=200)
+            oracle.gx = 9.5
+            oracle.set_paced_cells(5, 1, 0, 0)
+        elif w == 'cells':
+            s, li = _config3_cells(prec)
+            seg = 10.0
+            bpcs = 2 * 8 * wb + 3 * wb
+            desc = _config3_cells.__doc__
+        else:
+            s, li = _config3_cables(prec)
+            seg = 10.0
+            bpcs = 2 * 8 * wb + 2 * (4 + wb) + 2 * wb
+            desc = _config3_cables.__doc__
+        el, tm = _time_blocks(s, seg, li, args.steps, args.warmup)
+        n = s._ntotal
+        nsteps = int(round(seg / s._step_size))
+        value = args.steps * nsteps * n / el
+        avg_ms = tm['kernel_ms'] / max(tm['launches'], 1)
+        cpl = tm['cell_steps'] / max(tm['launches'], 1)
+        achieved = bpcs * cpl / (avg_ms * 1e-3) / 1e9
+        line.update({
+            'metric': 'cell-steps/s (%s)' % w, 'value': value,
+            'ms_per_step': el / args.steps * 1e3,
+            'dtype': 'f32' if prec == 32 else 'f64',
+            'config': {'workload': ' '.join(desc.split()), 'cells': n,
+                       'time_steps_per_step': nsteps,
+                       'driver': s.engine_driver()},
+            'roofline': {'bound': 'hbm', 'achieved': achieved,
+                         'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                         'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                         'kernel': 'mk_' + s.engine_driver(),
+                         'avg_launch_ms': avg_ms,
+                         'bytes_per_cell_step': bpcs,
+                         'cells_per_launch': cpl}})
+        if w == 'cells':
+            oracle = OracleSimulation(myokit.load_model('example'),
+                                      myokit.pacing.blocktrain(1000, 2),
+                                      ncells=n, diffusion=False)
+            for var, v in s._fields.items():
+                oracle.fields[var.qname()] = np.asarray(v, dtype=float)
+        elif w == 'cables':
+            oracle = OracleSimulation(myokit.load_model('example'),
+                                      myokit.pacing.blocktrain(1000, 2),
+                                      ncells=n)
+            oracle.connections = np.asarray(s._connections)
+            oracle.set_paced_cell_list(list(s._paced_cells))
+        if oracle is not None and not args.no_cpu:
+            oracle.dt = 0.005
+            sample = {'cable': 20.0, 'cells': 0.5, 'cables': 0.5}[w]
+            cv, cs, cel = _cpu_sample(oracle, sample, threads)
+            line['cpu_baseline'] = {
+                'value': cv, 'unit': 'cell-steps/s', 'cores': threads,
+                'kind': 'port',
+                'sample': 'CPU oracle (double-precision C, OpenMP) on the '
+                          'first %g ms (%d time steps) of the same workload, '
+                          '%.1f s' % (sample, cs, cel)}
+        return line
+    # adjoint workloads
+    from myokit_amd.adjoint import TissueAdjoint
+    if w == 'cable-adjoint':
+        s = _config4()
+        probes = np.linspace(25, 975, 20).astype(int)
+        window, K = args.window_ms or 50.0, 200
+        target = TissueAdjoint(_config4(101, 11.0), probes).trace(window)
+        desc = ('config 4: ' + ' '.join(_config4.__doc__.split())
+                + ' Loss over 20 probes against a trace of a hidden seeded'
+                ' parameter set (other fields, gx 11); checkpoints every'
+                ' 200 steps; window %g ms' % window)
+        prec = 64
+    else:
+        s, nedges = _config5(sets=args.sets)
+        n1 = s._ntotal // args.sets
+        probes = np.concatenate([np.arange(0, n1, n1 // 4) + k * n1
+                                 for k in range(args.sets)])
+        window, K = args.window_ms or 2.0, 200
+        target = np.full((int(round(window / 0.01)), len(probes)), -40.0)
+        desc = ('config 5: ' + ' '.join(_config5.__doc__.split())
+                + ' %d sets (%d cells, %d connections per set); loss over 4'
+                ' probes per set; FP32 state and VJP, FP64 loss and gradient'
+                ' reductions; window %g ms' % (args.sets, n1, nedges, window))
+        prec = 32
+    n = s._ntotal
+    adj = TissueAdjoint(s, probes, checkpoint_every=K)
+    nsteps = adj.n_steps(window)
+    # forward alone: the same window with one log row at its end
+    s.run_block(window, ['membrane.V'], log_interval=window)
+    s.reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.run_block(window, ['membrane.V'], log_interval=window)
+        s.reset()
+    fwd = (time.perf_counter() - t0) / args.steps
+    adj.loss_and_gradient(window, target)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        adj.loss_and_gradient(window, target)
+    both = (time.perf_counter() - t0) / args.steps
+    line.update({
+        'metric': 'fwd+adjoint vs fwd time ratio (%s)' % w,
+        'value': both / fwd, 'unit': 'x', 'higher_is_better': False,
+        'ms_per_step': both * 1e3,
+        'dtype': 'f32' if prec == 32 else 'f64',
+        'config': {'workload': desc, 'cells': n, 'time_steps': nsteps,
+                   'driver': s.engine_driver()},
+        'adjoint': {'fwd_ms': fwd * 1e3, 'fwd_adjoint_ms': both * 1e3,
+                    'phases_ms': adj.timing(),
+                    'reverse_kernel': adj.reverse_kernel()[0],
+                    'fwd_cell_steps_per_s': nsteps * n / fwd},
+    })
+    return line
+
+
 def adjoint_leg(s, window_ms, checkpoint_every, reps, vjp_precision=None):
     """Times loss_and_gradient (forward + discrete adjoint, gradients for gx
     and gy) over ``window_ms`` from the sheet's current state, 20 probe
@@ -156,6 +436,16 @@ def main():
     ap.add_argument('--adjoint-ms', type=float, default=20.0,
                     help='window of the fwd+adjoint timing (0: skip)')
     ap.add_argument('--adjoint-reps', type=int, default=2)
+    ap.add_argument('--workload', default='sheet',
+                    choices=['sheet', 'cable', 'cells', 'cables',
+                             'cable-adjoint', 'graph-adjoint'],
+                    help='sheet: the 1024x1024 sheet (default, the headline '
+                    'metric); cable: config 1; cells and cables: config 3; '
+                    'cable-adjoint: config 4; graph-adjoint: config 5')
+    ap.add_argument('--window-ms', type=float, default=0.0,
+                    help='adjoint workloads: simulated window')
+    ap.add_argument('--sets', type=int, default=32,
+                    help='graph-adjoint: parameter sets in the batch')
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -163,6 +453,12 @@ def main():
               ' path (see DESIGN.md for how the sheet would shard).',
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != 'sheet':
+        line = other_workload(args)
+        line['n_gpus'] = 1
+        print(json.dumps(line), flush=True)
+        return
 
     s = workload(args.nx, args.ny, args.precision)
     seg = args.segment_ms

@@ -244,7 +244,9 @@ class SimulationOpenCL:
         self._global = ['time', 'pace']
         self._engine_key = None
         self._device = 0
-        self._block_threads = 256
+        # threads per workgroup; None: chosen per layout (_threads)
+        self._block_threads = None
+        self._domains_cache = None
         self._tuning = {}
         self._options = {}
         self.last_run_stats = {}
@@ -339,9 +341,9 @@ class SimulationOpenCL:
             out = []
             for i, j, c in self._connections:
                 if i == x:
-                    out.append(j)
+                    out.append(int(j))
                 elif j == x:
-                    out.append(i)
+                    out.append(int(i))
             return out
         out = []
         if x > 0:
@@ -435,6 +437,10 @@ class SimulationOpenCL:
             raise ValueError(
                 'Connection list cannot be None. To unset connections, call '
                 ' set_conductance() with the new conductance value.')
+        if isinstance(connections, np.ndarray):
+            self._connections = self._connection_array(connections)
+            self._gx_field = self._gy_field = None
+            return
         conns = []
         doubles = set()
         for x in connections:
@@ -462,6 +468,40 @@ class SimulationOpenCL:
         self._connections = conns
         self._gx_field = self._gy_field = None
 
+    def _connection_array(self, a):
+        """set_connections for an (E, 3) array of (cell, cell, conductance)
+        rows: the checks and messages of the list form, vectorised for
+        graphs of millions of connections. Returns the rows with the lower
+        index first, as float64."""
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim != 2 or a.shape[1] != 3:
+            raise ValueError(
+                'Connections must be a list of 3-tuples'
+                ' (cell_index_1, cell_index_2, conductance).')
+        i = a[:, 0].astype(np.int64)
+        j = a[:, 1].astype(np.int64)
+        c = a[:, 2]
+        bad = (i == j) | (i < 0) | (j < 0) | (i >= self._nx) | (j >= self._nx)
+        if bad.any():
+            k = int(np.argmax(bad))
+            raise ValueError(
+                'Invalid connection: (' + str(int(i[k])) + ', '
+                + str(int(j[k])) + ', ' + str(c[k]) + ')')
+        lo, hi = np.minimum(i, j), np.maximum(i, j)
+        key = lo * self._nx + hi
+        order = np.argsort(key, kind='stable')
+        dup = np.nonzero(key[order][1:] == key[order][:-1])[0]
+        if len(dup):
+            k = int(order[dup[0] + 1])
+            raise ValueError(
+                'Duplicate connection: (' + str(int(lo[k])) + ', '
+                + str(int(hi[k])) + ', ' + str(c[k]) + ')')
+        if (c < 0).any():
+            k = int(np.argmax(c < 0))
+            raise ValueError('Invalid conductance: ' + str(c[k]))
+        return np.stack([lo.astype(np.float64), hi.astype(np.float64), c],
+                        axis=1)
+
     def set_constant(self, var, value):
         value = float(value)
         if isinstance(var, Variable):
@@ -471,6 +511,8 @@ class SimulationOpenCL:
             raise ValueError(
                 'The given variable <' + var.qname() + '> is not a literal.')
         self._model.set_value(var.qname(), value)
+        # the generated kernel inlines constants
+        self._sources.clear()
 
     def set_default_state(self, state, x=None, y=None):
         self._default_state = self._set_state(state, x, y,
@@ -668,10 +710,56 @@ class SimulationOpenCL:
             self._tuning['driver'] = driver
             self.set_engine_option('driver', names[driver])
 
+    def _graph_domains(self):
+        """(largest, count) of the connected components of the
+        set_connections graph (isolated cells count as components of one),
+        cached per graph."""
+        c = self._domains_cache
+        if c is not None and c[0] == self._graph_version:
+            return c[1]
+        from scipy.sparse import coo_matrix
+        from scipy.sparse.csgraph import connected_components
+        a = np.asarray(self._connections, dtype=np.float64).reshape(-1, 3)
+        n = self._ntotal
+        g = coo_matrix((np.ones(len(a)), (a[:, 0].astype(np.int64),
+                                          a[:, 1].astype(np.int64))),
+                       shape=(n, n))
+        k, labels = connected_components(g, directed=False)
+        r = (int(np.bincount(labels).max()), int(k))
+        self._domains_cache = (self._graph_version, r)
+        return r
+
+    def _threads(self):
+        """Threads per workgroup of the cells, block and grid kernels: the
+        value given to :meth:`set_tuning`, else 256, except where the
+        library will run the block layout (one workgroup per domain:
+        graphs whose components all fit a workgroup's LDS tile, and single
+        cables or sheets of at most 512 cells). There it is one thread per
+        cell of the largest domain (rounded up to a multiple of 64, at most
+        1024): one workgroup per cable leaves a GPU with a few waves per
+        CU, and more threads hide more latency (256 cables of 50-2000
+        cells: 3.99e10 cell-steps/s with 1024 threads, 2.05e10 with
+        256)."""
+        if self._block_threads is not None:
+            return self._block_threads
+        c = self._coupling()
+        if c == _native.COUPLING_NONE:
+            return 256
+        cap = 2048 if self._precision == DOUBLE_PRECISION else 4096
+        if c == _native.COUPLING_GRAPH:
+            largest, _ = self._graph_domains()
+            block = largest <= cap
+        else:
+            largest = self._ntotal
+            block = largest <= min(cap, 512)
+        if not block:
+            return 256
+        return int(min(1024, max(64, -(-largest // 64) * 64)))
+
     def tuning(self):
         """The current launch configuration (see :meth:`set_tuning`)."""
         return {'driver': self._tuning.get('driver', 'auto'),
-                'block_threads': self._block_threads,
+                'block_threads': self._threads(),
                 'sheet_cpt': self._tuning.get(
                     'sheet_cpt', _kernelgen.sheet_cpt(self._precision)),
                 'sheet_waves': self._tuning.get(
@@ -714,29 +802,40 @@ class SimulationOpenCL:
         return {'kernel_ms': ms.value, 'launches': nl.value,
                 'cell_steps': cs.value}
 
-    def _get_engine(self, quantities, adjoint=False, adjoint_precision=None):
+    def _kernel_source(self, quantities, adjoint=False,
+                       adjoint_precision=None):
+        """The generated kernel source and its CellModel, cached per
+        simulation by everything the generator reads (the model's values
+        too: ``set_constant`` clears the cache)."""
         fields = list(self._fields.keys())
         coupling = self._coupling()
         tune = self._tuning
+        threads = self._threads()
         gkey = (tuple(fields), tuple(quantities), int(coupling),
-                self._block_threads, self._native_math, bool(adjoint),
+                threads, self._native_math, bool(adjoint),
                 adjoint_precision, tuple(sorted(tune.items())))
         hit = self._sources.get(gkey)
-        if hit is None:
-            cm = _kernelgen.CellModel(self._model, self._bound_variables,
-                                      self._rl_states, fields, quantities)
-            src = _kernelgen.generate_kernel(
-                cm, self._precision, coupling, self._block_threads,
-                self._native_math, adjoint=adjoint,
-                adjoint_precision=adjoint_precision,
-                tile_cpt=tune.get('sheet_cpt'),
-                tile_waves=tune.get('sheet_waves'),
-                adjoint_waves=tune.get('adjoint_waves'))
-            if len(self._sources) >= 16:
-                self._sources.pop(next(iter(self._sources)))
-            self._sources[gkey] = (src, cm)
-        else:
-            src, cm = hit
+        if hit is not None:
+            return hit
+        cm = _kernelgen.CellModel(self._model, self._bound_variables,
+                                  self._rl_states, fields, quantities)
+        src = _kernelgen.generate_kernel(
+            cm, self._precision, coupling, threads,
+            self._native_math, adjoint=adjoint,
+            adjoint_precision=adjoint_precision,
+            tile_cpt=tune.get('sheet_cpt'),
+            tile_waves=tune.get('sheet_waves'),
+            adjoint_waves=tune.get('adjoint_waves'))
+        if len(self._sources) >= 16:
+            self._sources.pop(next(iter(self._sources)))
+        self._sources[gkey] = (src, cm)
+        return src, cm
+
+    def _get_engine(self, quantities, adjoint=False, adjoint_precision=None):
+        src, cm = self._kernel_source(quantities, adjoint, adjoint_precision)
+        fields = list(self._fields.keys())
+        coupling = self._coupling()
+
         opts = _kernelgen.compile_options(self._precision)
         key = (src, opts, self._ntotal, self._dims)
         if self._engine is not None and self._engine_key == key:
@@ -785,7 +884,7 @@ class SimulationOpenCL:
         cfg.i_vm = self._vm.index() if self._vm is not None else 0
         cfg.coupling = coupling
         cfg.driver = -1
-        cfg.block_threads = self._block_threads
+        cfg.block_threads = self._threads()
         cfg.nx = self._nx
         cfg.ny = self._ny
         cfg.device = self._device

@@ -41,7 +41,8 @@ def _key(sim):
     h.update(repr((sim._dims, sim._precision, sim._coupling(),
                    sorted(v.qname() for v in sim._fields),
                    sorted(s.qname() for s in sim._rl_states),
-                   bool(sim._connections))).encode())
+                   sim._connections is not None
+                   and len(sim._connections) > 0)).encode())
     try:
         h.update(_native.device_info(sim._device)['name'].encode())
     except Exception:       # pragma: no cover
@@ -55,7 +56,7 @@ def candidates(sim):
     out = []
     c = sim._coupling()
     n = sim._ntotal
-    bts = [128, 256, 512]
+    bts = [128, 256, 512, 1024]
     if c == _native.COUPLING_NONE:
         drivers = ['cells', 'grid']
     elif c == _native.COUPLING_GRAPH:

@@ -188,3 +188,81 @@ def test_monodomain_conductance(model):
     # g = D * A * k / ((k + 1) * chi * dx^2)
     assert s.monodomain_conductance(2, 3, 4, 0.5, 1) == pytest.approx(
         4 * 3 / (4 * 2 * 0.25))
+
+
+def test_connections_as_array(model):
+    # An (E, 3) array takes the vectorised path of set_connections (graphs
+    # of millions of connections); it stores what the list form stores and
+    # raises the same errors with the same messages.
+    s = myokit.SimulationOpenCL(model, ncells=10)
+    conns = [(0, 1, 9.5), (2, 1, 3.0), (5, 6, 1.0)]
+    s.set_connections(conns)
+    ref = np.array(s._connections, dtype=float)
+    s.set_connections(np.array(conns))
+    np.testing.assert_array_equal(np.asarray(s._connections), ref)
+    assert s.neighbors(1) == [0, 2]
+    for bad, msg in (([(0, 0, 1.0)], 'Invalid connection: \\(0, 0'),
+                     ([(0, 10, 1.0)], 'Invalid connection'),
+                     ([(0, 1, 1.0), (1, 0, 2.0)], 'Duplicate connection'),
+                     ([(0, 1, -1.0)], 'Invalid conductance')):
+        for form in (bad, np.array(bad)):
+            with pytest.raises(ValueError, match=msg):
+                s.set_connections(form)
+    with pytest.raises(ValueError, match='3-tuples'):
+        s.set_connections(np.zeros((4, 2)))
+
+
+def test_kernel_source_cache(model):
+    # A repeated run reuses its generated kernel source; set_constant (the
+    # generator inlines constants) and a change of fields or tuning make a
+    # new one.
+    s = myokit.SimulationOpenCL(model, ncells=8, rl=True)
+    q = [s._model.get('membrane.V')]
+    a, _ = s._kernel_source(q)
+    b, _ = s._kernel_source(q)
+    assert a is b
+    s.set_constant('ina.gNa', 17.0)
+    c, _ = s._kernel_source(q)
+    assert c is not a and c != a
+    assert s._kernel_source(q)[0] is c
+    s.set_field('ina.gNa', np.ones(8))
+    assert s._kernel_source(q)[0] != c
+    s.set_tuning(sheet_cpt=2)
+    d, _ = s._kernel_source(q)
+    assert s._kernel_source(q, adjoint=True)[0] != d
+
+
+def test_threads_per_workgroup_follow_the_layout(model):
+    # Unless set with set_tuning, the block layout gets one thread per cell
+    # of its largest domain (multiple of 64, at most 1024); other layouts
+    # 256.
+    s = myokit.SimulationOpenCL(model, ncells=200, precision=64)
+    assert s.tuning()['block_threads'] == 256          # one 200-cell cable
+    s = myokit.SimulationOpenCL(model, ncells=384)
+    assert s.tuning()['block_threads'] == 384
+    s = myokit.SimulationOpenCL(model, ncells=1000)
+    assert s.tuning()['block_threads'] == 256          # grid layout
+    s = myokit.SimulationOpenCL(model, ncells=(24, 16))
+    assert s.tuning()['block_threads'] == 384
+    s = myokit.SimulationOpenCL(model, ncells=(1024, 1024))
+    assert s.tuning()['block_threads'] == 256
+    s = myokit.SimulationOpenCL(model, ncells=4096, diffusion=False)
+    assert s.tuning()['block_threads'] == 256
+    # cables of 50 .. 2000 cells as one graph: largest component 2000
+    lengths, rows, off = [50, 2000, 700], [], 0
+    for L in lengths:
+        i = np.arange(off, off + L - 1)
+        rows.append(np.stack([i, i + 1, np.full(L - 1, 9.5)], axis=1))
+        off += L
+    s = myokit.SimulationOpenCL(model, ncells=off)
+    s.set_connections(np.concatenate(rows))
+    assert s._graph_domains() == (2000, 3)
+    assert s.tuning()['block_threads'] == 1024
+    s64 = myokit.SimulationOpenCL(model, ncells=off, precision=64)
+    s64.set_connections(np.concatenate(rows))
+    assert s64.tuning()['block_threads'] == 1024      # 2000 <= 2048
+    s.set_connections([(0, 1, 1.0)])                  # components 2, 1, ..
+    assert s._graph_domains() == (2, off - 1)
+    assert s.tuning()['block_threads'] == 64
+    s.set_tuning(block_threads=512)
+    assert s.tuning()['block_threads'] == 512
