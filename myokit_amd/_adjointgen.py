@@ -519,16 +519,27 @@ def _island(r, e, k, tab):
     for key in list(r64.bars):
         r64.bars[key] = 'isl_' + r.bars[key]
     # each distinct exp once (inside the branches of a piecewise the
-    # compiler would not merge repeated evaluations)
+    # compiler would not merge repeated evaluations), and exps of the same
+    # rate, exp(a x + m2) = exp(a x + m1) exp(m2 - m1), from one evaluation
+    import math
     exps = []
+    rates = {}
 
     def find(x):
         for y in x:
             find(y)
         if isinstance(x, ex.Exp) and x.code() not in r64.w.subst:
             name = 'ie' + str(k) + '_' + str(len(exps))
-            exps.append('const double ' + name + ' = '
-                        + r64.w.ex(x) + ';')
+            aff = r64.w.affine(x[0])
+            base = rates.get(aff[:2]) if aff is not None else None
+            if base is not None and abs(aff[2] - base[1]) <= 20:
+                code = '(' + base[0] + ' * ' + real_literal(
+                    math.exp(aff[2] - base[1]), DOUBLE_PRECISION) + ')'
+            else:
+                code = r64.w.ex(x)
+                if aff is not None:
+                    rates[aff[:2]] = (name, aff[2])
+            exps.append('const double ' + name + ' = ' + code + ';')
             r64.w.subst[x.code()] = name
     find(e)
     r64.out.extend(tab + line for line in exps)
@@ -543,12 +554,87 @@ def _island(r, e, k, tab):
     used = sorted(set(re.findall(r'\bisl_\w+', body)))
     jac = {u[4:]: 'J' + str(k) + '_' + u[4:] for u in used}
     lines = [tab + 'Real ' + jac[u[4:]] + ';' for u in used]
-    lines.append(tab + '{')
+    win = _island_window(r, e)
+    if win is not None:
+        lines.append(tab + 'if (' + win + ') {')
+    else:
+        lines.append(tab + '{')
     lines += [tab + tab + 'double ' + u + ' = 0.0;' for u in used]
     lines += [tab + line for line in body.split('\n') if line]
     lines += [tab + tab + jac[u[4:]] + ' = (Real)' + u + ';' for u in used]
+    if win is not None:
+        # Away from the singular point the single-precision rule is as
+        # accurate as the rest of the VJP
+        r32 = _Rev(r.cm, r.p)
+        r32.w.set_exp_members(r.w.exp_members)
+        r32._ntmp = r._ntmp
+        r32._inf = r._inf
+        for key in list(r32.bars):
+            r32.bars[key] = 'isf_' + r.bars[key]
+        r32.rev(e, '(Real)1')
+        r._ntmp = r32._ntmp
+        body32 = '\n'.join(r32.out)
+        lines.append(tab + '} else {')
+        lines += [tab + tab + 'Real isf_' + u[4:] + ' = (Real)0;'
+                  for u in used]
+        lines += [tab + line for line in body32.split('\n') if line]
+        lines += [tab + tab + jac[u[4:]] + ' = isf_' + u[4:] + ';'
+                  for u in used]
     lines.append(tab + '}')
     return lines, jac
+
+
+# Half-width, in units of the exponent's argument, of the window around a
+# removable singularity of the form N / (1 - exp(a x + m)) in which the
+# partial derivatives are computed in double precision. The single-precision
+# quotient rule loses ~6.5e-5 / u^2 relative at u mV from ina.m's point (a =
+# 0.1 per mV): 3e-7 at the window's edge (|a x + m| = 1.5, u = 15 mV).
+ISLAND_WINDOW = 1.5
+
+
+def _island_window(r, e):
+    """C condition (single precision) under which the island of ``e`` must
+    be computed in double precision, or None if it always must: every
+    cancelling factor of every denominator in ``e`` has the form 1 -
+    exp(a x + m) or exp(a x + m) - 1 with x one variable and a, m
+    literals, and the condition is |a x + m| < ISLAND_WINDOW for one of
+    them. Quotients whose cancelling factor is anything else (ik.xi's
+    V + 77) keep the island everywhere."""
+    conds = []
+
+    def factors(d):
+        if isinstance(d, ex.Multiply):
+            return factors(d[0]) + factors(d[1])
+        return [d]
+
+    def expm1_arg(f):
+        if isinstance(f, ex.Minus):
+            a, b = f
+            if isinstance(a, ex.Number) and a.eval() == 1 and \
+                    isinstance(b, ex.Exp):
+                return b[0]
+            if isinstance(b, ex.Number) and b.eval() == 1 and \
+                    isinstance(a, ex.Exp):
+                return a[0]
+        return None
+
+    def walk(x):
+        if isinstance(x, ex.Divide) and _may_cancel(x[1], r.cm):
+            for f in factors(x[1]):
+                if not _may_cancel(f, r.cm):
+                    continue
+                arg = expm1_arg(f)
+                aff = r.w.affine(arg) if arg is not None else None
+                if aff is None:
+                    return False
+                conds.append('__builtin_fabsf(__builtin_fmaf(' + aff[0]
+                             + ', ' + real_literal(aff[1], r.p) + ', '
+                             + real_literal(aff[2], r.p) + ')) < '
+                             + real_literal(ISLAND_WINDOW, r.p))
+        return all(walk(y) for y in x)
+    if r.p == DOUBLE_PRECISION or not walk(e) or not conds:
+        return None
+    return ' || '.join(sorted(set(conds)))
 
 
 def _mixed_source(cm):
@@ -597,7 +683,15 @@ def _vjp_source(cm, precision, islands=True):
         '{',
         tab + '(void)t; (void)dt; (void)pace; (void)idiff; (void)fv;',
     ]
-    lines.extend(forward_sweep(cm, w, tab))
+    fam = []
+    if precision != DOUBLE_PRECISION:
+        # the forward kernel's exponential families (one exp per family)
+        from ._kernelgen import exp_families
+        fam, members = exp_families(cm, w)
+        w.set_exp_members(members)
+    sweep = forward_sweep(cm, w, tab)
+    ns = len(cm.states)
+    lines.extend(sweep[:ns] + [tab + x for x in fam] + sweep[ns:])
     islands_jac = {}
     if islands and precision != DOUBLE_PRECISION:
         for var in cm.order:
