@@ -448,3 +448,38 @@ def test_reverse_sheet_tile_height_matches_header():
     src = _kernelgen.generate_kernel(cm, 32, 1, adjoint=True)
     assert '#define MK_ADJ_SHEET 1' in src
     assert '#define MK_ADJ_SWAVES 2' in src
+
+
+def test_island_window_edges(tmp_path):
+    # ina.m's alpha, 0.32 (V + 47.13) / (1 - exp(-0.1 (V + 47.13))), is
+    # differentiated in double precision only within 15 mV of its singular
+    # point (|-0.1 (V + 47.13)| < 1.5) and by the single-precision rule
+    # outside; ik.xi, whose cancelling factor is V + 77, keeps its island
+    # everywhere. On both sides of the window's edges and far from the
+    # singular point, d m' / dV of the single-precision VJP agrees with the
+    # double one to 1e-5 relative. Within 0.3 mV of the point the single-
+    # precision forward values themselves (1 - exp(-0.1 u) for small u) carry
+    # ~1e-5: 1e-4 there. (At the point itself, within ~1e-4 mV, the quotient
+    # rule cancels to second order in any precision, as alpha does in the
+    # forward step: the model's own conditioning there, not tested.)
+    from myokit_amd import _adjointgen
+    model = myokit.load_model('example')
+    cell = Cell(tmp_path, model, True, ['ina.gNa'])
+    src = generate_vjp_function(cell.cm, 32)
+    assert src.count('if (__builtin_fabsf(__builtin_fmaf(') == 1
+    assert _adjointgen.ISLAND_WINDOW == 1.5
+    fv = np.array([16.0])
+    names = [s.qname() for s in cell.cm.states]
+    iv, im = names.index('membrane.V'), names.index('ina.m')
+    lam = np.zeros(cell.ns)
+    lam[im] = 1.0
+    for dv in (-40, -15.2, -15.0, -14.8, -5, -0.3, -0.01, 0.01, 0.3, 5,
+               14.8, 15.0, 15.2, 40):
+        y = cell.y0.copy()
+        y[iv] = -47.13 + dv
+        y = y.astype(np.float32).astype(float)
+        m32 = cell.vjp(32, y, 1.0, 0.01, 0.0, 0.0, fv, lam)[0]
+        m64 = cell.vjp(64, y, float(np.float32(1.0)), float(np.float32(0.01)),
+                       0.0, 0.0, fv, lam)[0]
+        tol = 1e-4 if abs(dv) <= 0.3 else 1e-5
+        assert m32[iv] == pytest.approx(m64[iv], rel=tol), dv
