@@ -100,23 +100,36 @@ class TissueAdjoint:
         eng = self._engine()
         ev, nev = self._events()
         nf = len(sim._fields)
-        gf = np.zeros(max(nf, 1) * sim._ntotal)
+        # page-locked: the gradients come back by DMA, without page faults
+        gf = _native.pinned.empty((max(nf, 1) * sim._ntotal,), np.float64)
         gg = np.zeros(2)
         loss = ctypes.c_double(0)
+        probes = self._device_probes()
         self._check(_native.lib().mkh_adj_run(
             eng.h, sim._time, sim._time + duration, sim._step_size,
             _native.ptr(ev, ctypes.c_double), nev,
-            _native.ptr(self.probes, ctypes.c_longlong), len(self.probes),
+            _native.ptr(probes, ctypes.c_longlong), len(probes),
             _native.ptr(target, ctypes.c_double), target.shape[0],
             self.checkpoint_every, ctypes.byref(loss),
             _native.ptr(gf, ctypes.c_double),
             _native.ptr(gg, ctypes.c_double)), 'mkh_adj_run')
         shape = (sim._ny, sim._nx) if len(sim._dims) == 2 else (sim._nx,)
+        _, iperm = sim._cell_order()
         grads = {}
         for k, var in enumerate(sim._fields.keys()):
-            grads[var.qname()] = gf[k * sim._ntotal:(k + 1) * sim._ntotal] \
-                .reshape(shape).copy()
+            g = gf[k * sim._ntotal:(k + 1) * sim._ntotal]
+            # a copy either way (the pinned block goes back to its pool):
+            # the gather into the user's order or the slice's own
+            g = np.take(g, iperm) if iperm is not None else g.copy()
+            grads[var.qname()] = g.reshape(shape)
         return loss.value, grads, gg[0], gg[1]
+
+    def _device_probes(self):
+        """The probe cells in the device's cell order."""
+        _, iperm = self.sim._cell_order()
+        if iperm is None:
+            return self.probes
+        return np.ascontiguousarray(iperm[self.probes], dtype=np.int64)
 
     def timing(self):
         """Wall time in ms of the last ``loss_and_gradient``'s phases:
@@ -160,10 +173,11 @@ class TissueAdjoint:
         ev, nev = self._events()
         n = self.n_steps(duration)
         out = np.zeros((n, len(self.probes)))
+        probes = self._device_probes()
         self._check(_native.lib().mkh_adj_trace(
             eng.h, sim._time, sim._time + duration, sim._step_size,
             _native.ptr(ev, ctypes.c_double), nev,
-            _native.ptr(self.probes, ctypes.c_longlong), len(self.probes),
+            _native.ptr(probes, ctypes.c_longlong), len(probes),
             _native.ptr(out, ctypes.c_double), n), 'mkh_adj_trace')
         return out
 

@@ -1550,13 +1550,22 @@ struct Sim : SimBase {
             adj_ms[2] = f1 - recompute_ms;
         }
         /* Reductions in double precision, in cell / workgroup order */
-        std::vector<double> h(std::max<size_t>(npad, gfield.n));
+        std::vector<double> h(npad == n ? (size_t)npad
+                                        : std::max<size_t>(npad, gfield.n));
         if (gfield_out && cfg.n_field > 0) {
-            MKH_HIP(hipMemcpy(h.data(), gfield.p, gfield.n * sizeof(double),
-                              hipMemcpyDeviceToHost));
-            for (int k = 0; k < cfg.n_field; ++k)
-                for (long long c = 0; c < n; ++c)
-                    gfield_out[k * n + c] = h[(size_t)k * npad + c];
+            if (npad == n) {
+                /* same layout: straight into the caller's buffer */
+                MKH_HIP(hipMemcpy(gfield_out, gfield.p,
+                                  gfield.n * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+            } else {
+                MKH_HIP(hipMemcpy(h.data(), gfield.p,
+                                  gfield.n * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+                for (int k = 0; k < cfg.n_field; ++k)
+                    for (long long c = 0; c < n; ++c)
+                        gfield_out[k * n + c] = h[(size_t)k * npad + c];
+            }
         }
         double sx = 0, sy = 0, sl = 0;
         MKH_HIP(hipMemcpy(h.data(), gxa.p, nparts * sizeof(double),

@@ -154,8 +154,54 @@ class SimulationOpenCL:
         _native.check(_native.lib().mkh_sim_get_state(
             self._engine.h, _native.ptr(st, ctypes.c_double), len(st)),
             'get_state')
+        perm, iperm = self._cell_order()
+        if iperm is not None:
+            st = st.reshape(self._ntotal, self._nstate)[iperm].reshape(-1)
         self._state_arr = st
         self._dev_newer = False
+
+    def _cell_order(self):
+        """``(perm, iperm)``: the order in which the device holds the cells
+        of a set_connections graph (device cell k is the user's cell
+        perm[k], user cell c is device cell iperm[c]), or ``(None, None)``
+        for the user's own order. Graphs are held in reverse Cuthill-McKee
+        order, which puts neighbours close together in memory: the
+        diffusion current's and the adjoint's gathers of neighbour values
+        then touch few cache lines (a random geometric graph of 262,144
+        cells numbered at random has neighbours 87,000 indices apart on
+        average, 330 after reordering). The per-cell arithmetic, the order
+        of each cell's connections included, is the same in every order."""
+        if self._connections is None or len(self._connections) == 0:
+            return None, None
+        c = self._order_cache
+        if c is not None and c[0] == self._graph_version:
+            return c[1], c[2]
+        from scipy.sparse import coo_matrix
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        a = np.asarray(self._connections, dtype=np.float64).reshape(-1, 3)
+        n = self._ntotal
+        i = a[:, 0].astype(np.int64)
+        j = a[:, 1].astype(np.int64)
+        g = coo_matrix((np.ones(2 * len(a), dtype=np.int8),
+                        (np.concatenate([i, j]), np.concatenate([j, i]))),
+                       shape=(n, n)).tocsr()
+        perm = np.asarray(reverse_cuthill_mckee(g, symmetric_mode=True),
+                          dtype=np.int64)
+        iperm = np.empty(n, dtype=np.int64)
+        iperm[perm] = np.arange(n)
+        if np.array_equal(perm, np.arange(n)):
+            perm = iperm = None
+        self._order_cache = (self._graph_version, perm, iperm)
+        self._pcolmaps = {}
+        return perm, iperm
+
+    def _before_reorder(self):
+        """Called before the device's cell order may change (a new graph,
+        or a graph replaced by a stencil): a newer state on the device comes
+        back in the old order, and goes up again in the new one."""
+        if self._dev_newer and self._engine is not None:
+            self._download_state()
+        self._host_dirty = True
 
     def _touch(self, what='config'):
         # Marks device-side configuration as changed
@@ -175,6 +221,10 @@ class SimulationOpenCL:
         self._sources = {}
         # number of quantities -> column map of run_block (every cell)
         self._block_colmaps = {}
+        # device cell order of a set_connections graph (_cell_order) and
+        # column maps permuted to it
+        self._order_cache = None
+        self._pcolmaps = {}
         if not model.is_valid():
             model.validate()
         if model.has_interdependent_components():
@@ -384,6 +434,9 @@ class SimulationOpenCL:
             raise ValueError('Invalid conductance gy=' + str(gy))
         self._gx, self._gy = gx, gy
         self._gx_field = self._gy_field = None
+        if getattr(self, '_connections', None) is not None:
+            self._before_reorder()
+            self._graph_version += 1
         self._connections = None
 
     def set_conductance_field(self, gx, gy=None):
@@ -424,9 +477,15 @@ class SimulationOpenCL:
             self._gy_field = None
         else:
             self._gy_field = list(gy.reshape(self._nx * (self._ny - 1)))
+        if getattr(self, '_connections', None) is not None:
+            self._before_reorder()
+            self._graph_version += 1
         self._connections = None
 
     def set_connections(self, connections):
+        if self._diffusion_enabled and len(self._dims) == 1 and \
+                connections is not None:
+            self._before_reorder()
         self._touch('graph')
         if not self._diffusion_enabled:
             raise RuntimeError(
@@ -922,6 +981,10 @@ class SimulationOpenCL:
                              dtype=np.float64).reshape(-1, 3)
                 ci = np.ascontiguousarray(a[:, 0], dtype=np.int64)
                 cj = np.ascontiguousarray(a[:, 1], dtype=np.int64)
+                _, iperm = self._cell_order()
+                if iperm is not None:
+                    ci = np.ascontiguousarray(iperm[ci])
+                    cj = np.ascontiguousarray(iperm[cj])
                 cg = np.ascontiguousarray(a[:, 2], dtype=np.float64)
                 _native.check(L.mkh_sim_set_connections(
                     h, _native.ptr(ci, ctypes.c_longlong),
@@ -929,30 +992,44 @@ class SimulationOpenCL:
                     _native.ptr(cg, ctypes.c_double), len(ci)),
                     'set_connections')
                 eng.graph_version = self._graph_version
+            perm, iperm = self._cell_order()
+            paced = self._paced_cells
+            if perm is not None and type(paced) == tuple:
+                # a rectangle of a 1d graph is the cells x .. x + nx - 1;
+                # in the device's order it is a list
+                nx, ny, x, y = paced
+                paced = list(range(max(0, x), min(self._ntotal, x + nx))) \
+                    if y <= 0 < y + ny else []
             if not self._diffusion_enabled:
                 _native.check(L.mkh_sim_set_pacing_all(h), 'pacing')
-            elif type(self._paced_cells) == tuple:
-                nx, ny, x, y = self._paced_cells
+            elif type(paced) == tuple:
+                nx, ny, x, y = paced
                 _native.check(L.mkh_sim_set_pacing_rect(h, nx, ny, x, y),
                               'pacing')
-            elif self._paced_cells:
-                cells = np.ascontiguousarray(self._paced_cells,
-                                             dtype=np.int64)
+            elif paced:
+                cells = np.ascontiguousarray(paced, dtype=np.int64)
+                if iperm is not None:
+                    cells = np.ascontiguousarray(iperm[cells])
                 _native.check(L.mkh_sim_set_pacing_list(
                     h, _native.ptr(cells, ctypes.c_longlong), len(cells)),
                     'pacing')
             else:
                 _native.check(L.mkh_sim_set_pacing_none(h), 'pacing')
             if self._fields:
-                fd = np.ascontiguousarray(
-                    np.stack(list(self._fields.values()), axis=1).reshape(-1),
-                    dtype=np.float64)
+                fd = np.stack(list(self._fields.values()), axis=1)
+                if perm is not None:
+                    fd = fd[perm]
+                fd = np.ascontiguousarray(fd.reshape(-1), dtype=np.float64)
                 _native.check(L.mkh_sim_set_fields(
                     h, _native.ptr(fd, ctypes.c_double), len(fd)),
                     'set_fields')
             eng.config_version = self._config_version
         if self._host_dirty:
-            st = np.ascontiguousarray(self._state_arr, dtype=np.float64)
+            st = np.asarray(self._state_arr, dtype=np.float64)
+            perm, _ = self._cell_order()
+            if perm is not None:
+                st = st.reshape(self._ntotal, self._nstate)[perm].reshape(-1)
+            st = np.ascontiguousarray(st)
             _native.check(L.mkh_sim_set_state(
                 h, _native.ptr(st, ctypes.c_double), len(st)), 'set_state')
             self._host_dirty = False
@@ -1084,6 +1161,18 @@ class SimulationOpenCL:
         self._configure(eng)
         L = _native.lib()
         h = eng.h
+        perm, _ = self._cell_order()
+        if perm is not None:
+            # columns of the device's cells; cached per column map
+            key = id(colmap)
+            hit = self._pcolmaps.get(key)
+            if hit is None or hit[0] is not colmap:
+                hit = (colmap, np.ascontiguousarray(
+                    colmap.reshape(len(quantities), -1)[:, perm]))
+                if len(self._pcolmaps) >= 8:
+                    self._pcolmaps.clear()
+                self._pcolmaps[key] = hit
+            colmap = hit[1]
         _native.check(L.mkh_sim_set_logging(
             h, _native.ptr(np.ascontiguousarray(colmap.reshape(-1)),
                            ctypes.c_int),

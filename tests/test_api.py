@@ -266,3 +266,29 @@ def test_threads_per_workgroup_follow_the_layout(model):
     assert s.tuning()['block_threads'] == 64
     s.set_tuning(block_threads=512)
     assert s.tuning()['block_threads'] == 512
+
+
+def test_graph_cell_order(model):
+    # set_connections graphs live on the device in reverse Cuthill-McKee
+    # order: a permutation and its inverse, recomputed for a new graph and
+    # dropped with the graph.
+    rng = np.random.default_rng(4)
+    n = 300
+    s = myokit.SimulationOpenCL(model, ncells=n)
+    assert s._cell_order() == (None, None)
+    pts = rng.uniform(0, 1, (n, 2))
+    from scipy.spatial import cKDTree
+    pairs = cKDTree(pts).query_pairs(0.12, output_type='ndarray')
+    rows = np.concatenate([pairs, np.full((len(pairs), 1), 5.0)], axis=1)
+    s.set_connections(rows)
+    perm, iperm = s._cell_order()
+    assert sorted(perm) == list(range(n))
+    np.testing.assert_array_equal(iperm[perm], np.arange(n))
+    i, j = pairs[:, 0], pairs[:, 1]
+    assert np.mean(np.abs(iperm[i] - iperm[j])) < 0.5 * np.mean(np.abs(i - j))
+    assert s._cell_order()[0] is perm                 # cached
+    s.set_connections([(0, 1, 1.0), (1, 2, 1.0)])
+    p2, _ = s._cell_order()
+    assert p2 is None or p2 is not perm
+    s.set_conductance(9.5)
+    assert s._cell_order() == (None, None)
