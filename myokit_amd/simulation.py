@@ -1168,7 +1168,7 @@ class SimulationOpenCL:
             hit = self._pcolmaps.get(key)
             if hit is None or hit[0] is not colmap:
                 hit = (colmap, np.ascontiguousarray(
-                    colmap.reshape(len(quantities), -1)[:, perm]))
+                    colmap.reshape(len(quantities), self._ntotal)[:, perm]))
                 if len(self._pcolmaps) >= 8:
                     self._pcolmaps.clear()
                 self._pcolmaps[key] = hit

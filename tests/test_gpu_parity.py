@@ -304,3 +304,56 @@ def test_fp64_persistent_sheet_bit_exact(shape):
     assert P['sim'].engine_driver() == 'sheet'
     assert _report('fp64 sheet ' + str(shape), P, O) <= FP64_TOL
     assert np.max(np.abs(P['state'] - O['state'])) <= FP64_TOL
+
+
+def test_fp64_graph_cell_order_boundaries():
+    # set_connections graphs live on the device in reverse Cuthill-McKee
+    # order; every per-cell array is permuted where it crosses the C ABI.
+    # A random geometric graph (the order far from the identity) with what
+    # crosses that boundary: a paced rectangle (the first 5 cells), a
+    # per-cell field, one cell's state set before the run, the state read
+    # back after it, and a second graph mid-simulation (the state comes back
+    # in the old order and goes up in the new one). Bit for bit against the
+    # oracle, which keeps the user's order.
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(12)
+    n = 400
+    pts = rng.uniform(0, 1, (n, 2))
+
+    def graph(r, seed):
+        g = np.random.default_rng(seed)
+        pairs = cKDTree(pts).query_pairs(r, output_type='ndarray')
+        return [(int(i), int(j), float(g.uniform(5, 12))) for i, j in pairs]
+    c1, c2 = graph(0.09, 1), graph(0.07, 2)
+    gna = 16.0 * rng.uniform(0.8, 1.2, n)
+    sc = Scenario('lr1991', n, dt=0.005, events=[(1, 0, 2, 1000, 0)],
+                  paced=(5,), connections=c1, fields={'ina.gNa': gna})
+    s = sc.product(myokit.DOUBLE_PRECISION)
+    perm, _ = s._cell_order()
+    assert perm is not None and not np.array_equal(perm, np.arange(n))
+    o = sc.oracle()
+    y = np.array(s.state()).reshape(n, -1)
+    y[17, 0] = -30.0
+    s.set_state(y[17], 17)
+    o.state = o.state.reshape(n, -1).copy()
+    o.state[17, 0] = -30.0
+    o.state = o.state.reshape(-1)
+    keys = [str(c) + '.membrane.V' for c in range(n)]
+    d = s.run(10.0, log=['membrane.V'], log_interval=0.5)
+    r = o.run(10.0, log_vars=['membrane.V'], log_interval=0.5)
+    V = np.stack([np.asarray(d[k]) for k in keys], axis=1)
+    print('graph order: max|dV| %g, state %g' % (
+        np.max(np.abs(V - r['membrane.V'])),
+        np.max(np.abs(np.array(s.state()) - o.state))), flush=True)
+    np.testing.assert_array_equal(V, r['membrane.V'])
+    np.testing.assert_array_equal(np.array(s.state()), o.state)
+    # a second graph: new device order, the run continues
+    s.run(2.0, log=myokit.LOG_NONE)
+    o.run(2.0, log_vars=['membrane.V'], log_interval=100.0)
+    s.set_connections(c2)
+    o.connections = [(min(i, j), max(i, j), g) for i, j, g in c2]
+    d = s.run(6.0, log=['membrane.V'], log_interval=0.5)
+    r = o.run(6.0, log_vars=['membrane.V'], log_interval=0.5)
+    V = np.stack([np.asarray(d[k]) for k in keys], axis=1)
+    np.testing.assert_array_equal(V, r['membrane.V'])
+    np.testing.assert_array_equal(np.array(s.state()), o.state)
