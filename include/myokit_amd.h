@@ -238,6 +238,16 @@ int mkh_adj_timing(void* sim, double* ms3);
 #define MKH_ADJ_REV_SHEET 2
 int mkh_adj_reverse(void* sim, int* kind, long long* launches);
 
+/* The caller's cell order for mkh_adj_run's field gradients: caller cell c
+ * is device cell order[c] (0 <= order[c] < n_cells, n_cells = the
+ * simulation's cells); the gradients are gathered into that order on the
+ * device before they are copied out. order NULL or n_cells 0: the device's
+ * order is the caller's. No counterpart in the reference, whose cells keep
+ * the caller's order; for callers that renumber the cells of a connection
+ * graph for locality. */
+int mkh_adj_set_cell_order(void* sim, const long long* order,
+                           long long n_cells);
+
 /* The forward quantity of mkh_adj_run's loss: V at the probe cells after
  * each of the N time steps of tmin..tmax, into trace[s * n_probes + p]
  * (n_rows >= N). Leaves the state unchanged. No counterpart in the

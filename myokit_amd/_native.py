@@ -84,6 +84,7 @@ SIGNATURES = {
                          _pd, _pd, _pd]),
     'mkh_adj_timing': (_i, [_vp, _pd]),
     'mkh_adj_reverse': (_i, [_vp, _pi, _pll]),
+    'mkh_adj_set_cell_order': (_i, [_vp, _pll, _ll]),
     'mkh_sim_copy_state': (_i, [_vp, _vp]),
     'mkh_ft_create': (_i, [_vp, _vp, _ll, _ll, _d,
                            ctypes.POINTER(ctypes.c_void_p)]),

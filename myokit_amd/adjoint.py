@@ -105,6 +105,16 @@ class TissueAdjoint:
         gg = np.zeros(2)
         loss = ctypes.c_double(0)
         probes = self._device_probes()
+        _, iperm = sim._cell_order()
+        if getattr(eng, 'grad_order', None) is not iperm:
+            # the device gathers the field gradients into the caller's cell
+            # order; uploaded once per cell order and engine
+            self._check(_native.lib().mkh_adj_set_cell_order(
+                eng.h, None if iperm is None
+                else _native.ptr(iperm, ctypes.c_longlong),
+                0 if iperm is None else len(iperm)),
+                'mkh_adj_set_cell_order')
+            eng.grad_order = iperm
         self._check(_native.lib().mkh_adj_run(
             eng.h, sim._time, sim._time + duration, sim._step_size,
             _native.ptr(ev, ctypes.c_double), nev,
@@ -114,14 +124,12 @@ class TissueAdjoint:
             _native.ptr(gf, ctypes.c_double),
             _native.ptr(gg, ctypes.c_double)), 'mkh_adj_run')
         shape = (sim._ny, sim._nx) if len(sim._dims) == 2 else (sim._nx,)
-        _, iperm = sim._cell_order()
         grads = {}
         for k, var in enumerate(sim._fields.keys()):
-            g = gf[k * sim._ntotal:(k + 1) * sim._ntotal]
-            # a copy either way (the pinned block goes back to its pool):
-            # the gather into the user's order or the slice's own
-            g = np.take(g, iperm) if iperm is not None else g.copy()
-            grads[var.qname()] = g.reshape(shape)
+            # views of the page-locked block, which goes back to its pool
+            # once every one of them is gone (no copy, no fresh pages)
+            grads[var.qname()] = gf[
+                k * sim._ntotal:(k + 1) * sim._ntotal].reshape(shape)
         return loss.value, grads, gg[0], gg[1]
 
     def _device_probes(self):

@@ -301,6 +301,23 @@ mk_adj_probe(const MkArgs a, const MkAdj j, const int s)
         (double)j.yn[(long long)MK_IVM * a.npad + j.probes[q]];
 }
 
+/* The field gradients in the caller's cell order, for the download:
+ * dst[k * n + c] = src[k * npad + order[c]], k < nf, c < n (the host checks
+ * 0 <= order[c] < n). */
+extern "C" __global__ void __launch_bounds__(256)
+mk_adj_gather(const double* __restrict__ src,
+              const long long* __restrict__ order, double* __restrict__ dst,
+              const long long n, const long long npad, const int nf)
+{
+    const long long total = n * nf;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total;
+         i += stride) {
+        const long long k = i / n;
+        dst[i] = src[k * npad + order[i - k * n]];
+    }
+}
+
 #if MK_ADJ_SHEET
 /* ------------------------------------------------------------------------
  * Persistent reverse sweep on a 2D stencil sheet: every reverse step of a

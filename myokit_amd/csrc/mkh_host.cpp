@@ -62,7 +62,8 @@ struct Program {
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
     hipFunction_t f_sheet = nullptr, f_junction = nullptr;
     hipFunction_t f_adj_step = nullptr, f_adj_block = nullptr,
-                  f_adj_probe = nullptr, f_adj_sheet = nullptr;
+                  f_adj_probe = nullptr, f_adj_sheet = nullptr,
+                  f_adj_gather = nullptr;
     std::vector<char> code;
 };
 
@@ -221,6 +222,7 @@ struct SimBase {
                         int, double*, double*, double*, double*) = 0;
     virtual int adj_timing(double*) = 0;
     virtual int adj_reverse(int*, long long*) = 0;
+    virtual int set_cell_order(const long long*, long long) = 0;
     virtual int copy_state_from(SimBase* src) = 0;
 };
 
@@ -1553,7 +1555,27 @@ struct Sim : SimBase {
         std::vector<double> h(npad == n ? (size_t)npad
                                         : std::max<size_t>(npad, gfield.n));
         if (gfield_out && cfg.n_field > 0) {
-            if (npad == n) {
+            if (cell_order.p && prog->f_adj_gather) {
+                /* gathered into the caller's order on the device, then
+                 * one copy into the caller's buffer */
+                const long long tot = n * cfg.n_field;
+                MKH_HIP(gsorted.alloc((size_t)tot));
+                const double* src = gfield.p;
+                const long long* ord = cell_order.p;
+                double* dst = gsorted.p;
+                long long nn = n, np = npad;
+                int nf = cfg.n_field;
+                void* params[] = {&src, &ord, &dst, &nn, &np, &nf};
+                const unsigned gg = (unsigned)std::min<long long>(
+                    (tot + 255) / 256, 16384);
+                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_gather, gg, 1, 1,
+                                              256, 1, 1, 0, stream, params,
+                                              nullptr));
+                MKH_HIP(hipMemcpyAsync(gfield_out, dst,
+                                       (size_t)tot * sizeof(double),
+                                       hipMemcpyDeviceToHost, stream));
+                MKH_HIP(hipStreamSynchronize(stream));
+            } else if (npad == n) {
                 /* same layout: straight into the caller's buffer */
                 MKH_HIP(hipMemcpy(gfield_out, gfield.p,
                                   gfield.n * sizeof(double),
@@ -1596,6 +1618,29 @@ struct Sim : SimBase {
     {
         *kind = adj_rev_kind;
         *launches = adj_rev_launches;
+        return 0;
+    }
+
+    /* Device cell order[c] of the caller's cell c, for the field gradients
+     * of adj_run (empty: the caller's order is the device's). */
+    DevBuf<long long> cell_order;
+    DevBuf<double> gsorted;
+
+    int set_cell_order(const long long* order, long long m) override
+    {
+        if (!order || m == 0) {
+            cell_order.release();
+            gsorted.release();
+            return 0;
+        }
+        if (m != n)
+            return fail(MKH_E_ARG, "cell order: one entry per cell needed");
+        for (long long c = 0; c < m; ++c)
+            if (order[c] < 0 || order[c] >= n)
+                return fail(MKH_E_ARG, "cell order: index out of range");
+        MKH_HIP(cell_order.alloc((size_t)n));
+        MKH_HIP(hipMemcpy(cell_order.p, order, (size_t)n * sizeof(long long),
+                          hipMemcpyHostToDevice));
         return 0;
     }
 
@@ -1841,9 +1886,11 @@ int mkh_program_build(const char* source, const char* options,
             hipModuleGetFunction(&prog->f_adj_block, prog->module,
                                  "mk_adj_block") != hipSuccess ||
             hipModuleGetFunction(&prog->f_adj_probe, prog->module,
-                                 "mk_adj_probe") != hipSuccess) {
+                                 "mk_adj_probe") != hipSuccess ||
+            hipModuleGetFunction(&prog->f_adj_gather, prog->module,
+                                 "mk_adj_gather") != hipSuccess) {
             prog->f_adj_step = prog->f_adj_block = prog->f_adj_probe =
-                nullptr;
+                prog->f_adj_gather = nullptr;
             (void)hipGetLastError();
         }
         /* the persistent reverse sheet: stencil kernels whose tile fits */
@@ -2122,6 +2169,14 @@ int mkh_adj_reverse(void* sim, int* kind, long long* launches)
     CHECK_SIM();
     if (!kind || !launches) return fail(MKH_E_ARG, "null argument");
     return SIM->adj_reverse(kind, launches);
+}
+
+int mkh_adj_set_cell_order(void* sim, const long long* order,
+                           long long n_cells)
+{
+    CHECK_SIM();
+    if (n_cells < 0) return fail(MKH_E_ARG, "bad argument");
+    return SIM->set_cell_order(order, n_cells);
 }
 
 int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,

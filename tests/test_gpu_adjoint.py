@@ -392,3 +392,53 @@ def test_adjoint_persistent_sheet_fp32(shape, K):
         assert a == pytest.approx(x3, rel=1e-5)
     for a in (y1, y2):
         assert a == pytest.approx(y3, rel=1e-5)
+
+
+def test_adjoint_graph_cell_order_fd_fp64():
+    # A random geometric graph is held on the device in another cell order;
+    # the field gradients are gathered back into the caller's numbering on
+    # the device. Central differences of the loss in gNa of single cells,
+    # numbered as the caller numbers them, pin that mapping; then again
+    # after set_connections changes the graph (and the device order).
+    from scipy.spatial import cKDTree
+    n = 300
+    rng = np.random.default_rng(17)
+    pts = rng.uniform(0, 1, (n, 2))
+    m = myokit.load_model('example')
+    s = myokit.SimulationOpenCL(m, myokit.pacing.blocktrain(1000, 2),
+                                ncells=n, precision=64)
+    s.set_step_size(0.005)
+
+    def graph(r):
+        pairs = cKDTree(pts).query_pairs(r, output_type='ndarray')
+        return [(int(i), int(j), float(g)) for (i, j), g in
+                zip(pairs, rng.uniform(5, 12, len(pairs)))]
+    paced = [int(c) for c in np.nonzero(pts[:, 0] < 0.15)[0]]
+    s.set_paced_cell_list(paced)
+    gna = 16 * rng.uniform(0.8, 1.2, n)
+    s.set_field('ina.gNa', gna)
+    probes = [int(np.argmin(np.abs(pts[:, 0] - 0.3))),
+              int(np.argmin(np.abs(pts[:, 0] - 0.5)))]
+    dur = 4.0
+    for r in (0.09, 0.11):
+        s.set_connections(graph(r))
+        perm, iperm = s._cell_order()
+        assert perm is not None and not np.array_equal(perm, np.arange(n))
+        adj = TissueAdjoint(s, probes, checkpoint_every=300)
+        target = np.full((adj.n_steps(dur), 2), -45.0)
+        loss, g, _, _ = adj.loss_and_gradient(dur, target)
+        g = np.array(g['ina.gNa'])
+        # probe cells and the largest gradient elsewhere
+        others = np.argsort(-np.abs(g))
+        cells = probes + [int(c) for c in others if c not in probes][:1]
+        for c in cells:
+            def lc(v, c=c):
+                f = gna.copy()
+                f[c] = v
+                s.set_field('ina.gNa', f)
+                return adj.loss_and_gradient(dur, target)[0]
+            d = fd(lc, gna[c], 1e-6 * gna[c])
+            s.set_field('ina.gNa', gna)
+            print('graph r %g cell %d (device %d) dL/dgNa %.10g fd %.10g'
+                  % (r, c, iperm[c], g[c], d), flush=True)
+            assert rel(g[c], d) < 1e-5
