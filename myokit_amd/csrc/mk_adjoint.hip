@@ -89,7 +89,32 @@ __device__ __forceinline__ Real mk_adj_couple(
     }
 #elif MK_COUPLING == 3
     const Real ac = ab[self], v = V1[self];
-    for (int e = a.csr_ptr[row]; e < a.csr_ptr[row + 1]; ++e) {
+    int e = a.csr_ptr[row];
+    const int e1 = a.csr_ptr[row + 1];
+    /* four connections per round, loads first, sums in list order (as in
+     * mk_graph) */
+    for (; e + 4 <= e1; e += 4) {
+        int nb[4];
+        Real gg[4], an[4], vn[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int t = a.csr_idx[e + k];
+            nb[k] = t < 0 ? ~t : t;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gg[k] = a.csr_g[e + k];
+            an[k] = ab[nb[k]];
+            vn[k] = V1[nb[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            lv += gg[k] * (ac - an[k]);
+            cx += 0.5 * (double)gg[k] * ((double)ac - (double)an[k])
+                  * ((double)v - (double)vn[k]);
+        }
+    }
+    for (; e < e1; ++e) {
         int nb = a.csr_idx[e];
         if (nb < 0) nb = ~nb;
         const Real g = a.csr_g[e];
