@@ -126,6 +126,57 @@ __device__ __forceinline__ Real mk_adj_couple(
     return lv;
 }
 
+#if MK_COUPLING == 3
+/* mk_adj_couple and mk_graph in one walk of graph row c (the one-step
+ * kernel: row = self = c): returns the coupling transpose at abar_{s+1}
+ * (ab) and V(y_{s+1}) (V1), adds the cell's dL/dscale terms to cx, and
+ * sets idiff to the forward current at V(y_s) (V0). Each sum takes the
+ * same additions in the same order as in the separate walks. */
+__device__ __forceinline__ Real mk_adj_couple_graph(
+    const MkArgs& a, const Real* __restrict__ V1, const Real* __restrict__ ab,
+    const Real* __restrict__ V0, int c, double& cx, Real& idiff)
+{
+    const Real ac = ab[c], v = V1[c], v0 = V0[c];
+    Real lv = (Real)0, id = (Real)0;
+    int e = a.csr_ptr[c];
+    const int e1 = a.csr_ptr[c + 1];
+    for (; e + 4 <= e1; e += 4) {
+        int t[4];
+        Real gg[4], an[4], vn[4], wn[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = a.csr_idx[e + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int nb = t[k] < 0 ? ~t[k] : t[k];
+            gg[k] = a.csr_g[e + k];
+            an[k] = ab[nb];
+            vn[k] = V1[nb];
+            wn[k] = V0[nb];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            lv += gg[k] * (ac - an[k]);
+            cx += 0.5 * (double)gg[k] * ((double)ac - (double)an[k])
+                  * ((double)v - (double)vn[k]);
+            if (t[k] >= 0) id = id + gg[k] * (v0 - wn[k]);
+            else id = id - gg[k] * (wn[k] - v0);
+        }
+    }
+    for (; e < e1; ++e) {
+        const int t = a.csr_idx[e];
+        const int nb = t < 0 ? ~t : t;
+        const Real g = a.csr_g[e];
+        lv += g * (ac - ab[nb]);
+        cx += 0.5 * (double)g * ((double)ac - (double)ab[nb])
+              * ((double)v - (double)V1[nb]);
+        if (t >= 0) id = id + g * (v0 - V0[nb]);
+        else id = id - g * (V0[nb] - v0);
+    }
+    idiff = id;
+    return lv;
+}
+#endif
+
 /* Loss term dL/dV at cell c for V(y_{s+1}) = v, target row `row`. */
 __device__ __forceinline__ Real mk_adj_loss(const MkAdj& j, int c, Real v,
                                             int row)
@@ -183,7 +234,17 @@ mk_adj_step(const MkArgs a, const MkAdj j, const int s)
         }
         const Real* __restrict__ V1 = j.yn1 + MK_IVM * np;
         Real lv = mk_adj_loss(j, c, V1[c], j.target_row);
+#if MK_COUPLING == 3
+        /* both walks of the cell's connections in one */
+        Real idiff0 = (Real)0;
+        const bool fused = j.ab_in && j.yn;
+        if (fused)
+            lv += mk_adj_couple_graph(a, V1, j.ab_in, j.yn + MK_IVM * np, c,
+                                      cx, idiff0);
+        else if (j.ab_in)
+#else
         if (j.ab_in)
+#endif
             lv += mk_adj_couple(a, V1, j.ab_in, c, c, c, ix, iy, nx, ny, cx,
                                 cy);
         lam[MK_IVM] += lv;
@@ -194,8 +255,21 @@ mk_adj_step(const MkArgs a, const MkAdj j, const int s)
             for (int i = 0; i < MK_NS; ++i) y[i] = yp[i * np];
 #pragma unroll
             for (int k = 0; k < MK_NF; ++k) fv[k] = a.fields[k * np + c];
+#if MK_COUPLING == 3
+            if (!fused)
+                idiff0 = mk_graph(j.yn + MK_IVM * np, a.csr_ptr, a.csr_idx,
+                                  a.csr_g, c, c);
+            {
+                const MkStep& p = a.steps[s];
+                const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace
+                                                            : (Real)0;
+                mk_cell_vjp(y, p.t, p.dt, pace, idiff0, fv, lam, mu, &abo,
+                            fb);
+            }
+#else
             mk_adj_cell(a, a.steps[s], y, fv, j.yn + MK_IVM * np, c, c, c, ix,
                         iy, nx, ny, lam, mu, abo, fb);
+#endif
             Real* __restrict__ mo = j.mu_out + c;
 #pragma unroll
             for (int i = 0; i < MK_NS; ++i) mo[i * np] = mu[i];
