@@ -451,8 +451,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU'
-              ' path (see DESIGN.md for how the sheet would shard).',
+        print('bench.py runs on one GPU only: the engine has no multi-GPU'
+              ' path (a sheet would shard by rows, one band per GPU, with'
+              ' one halo row of V exchanged per time step).',
               file=sys.stderr)
         sys.exit(2)
 
