@@ -114,27 +114,27 @@ __device__ __forceinline__ Real mk_graph(
     const int e1 = ptr[row + 1];
     /* four connections per round: their index, conductance and neighbour
      * loads are in flight together (one dependent load chain per round
-     * instead of per connection); the additions keep the list order */
-    for (; e + 4 <= e1; e += 4) {
+     * instead of per connection); the last round is masked; the additions
+     * keep the list order */
+    for (; e < e1; e += 4) {
         int j[4];
         Real gg[4], vj[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) j[k] = idx[e + k];
+        for (int k = 0; k < 4; ++k) j[k] = e + k < e1 ? idx[e + k] : 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            gg[k] = g[e + k];
-            vj[k] = V[j[k] >= 0 ? j[k] : ~j[k]];
+            if (e + k < e1) {
+                gg[k] = g[e + k];
+                vj[k] = V[j[k] >= 0 ? j[k] : ~j[k]];
+            }
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (j[k] >= 0) i = i + gg[k] * (v - vj[k]);
-            else i = i - gg[k] * (vj[k] - v);
+            if (e + k < e1) {
+                if (j[k] >= 0) i = i + gg[k] * (v - vj[k]);
+                else i = i - gg[k] * (vj[k] - v);
+            }
         }
-    }
-    for (; e < e1; ++e) {
-        const int j = idx[e];
-        if (j >= 0) i = i + g[e] * (v - V[j]);
-        else i = i - g[e] * (V[~j] - v);
     }
     return i;
 }
