@@ -80,18 +80,47 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
     return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
+S2_TIME = 320.0
+
+
 def workload(nx, ny, precision):
+    """The sheet with S1 (2 ms, left 4 columns, t = 0) and S2 (2 ms, lower
+    left quadrant, t = 320 ms) in one protocol. The paced region is S1's
+    until the run reaches 320 ms, where ``advance`` switches it to the
+    quadrant, as a user of the reference does with run(320) followed by
+    set_paced_cells (openclsim.py:1534-1629)."""
     import myokit_amd as myokit
     m = myokit.load_model('example')
     p = myokit.Protocol()
     p.schedule(1, 0, 2)          # S1 at t = 0, 2 ms
-    p.schedule(1, 320, 2)        # S2 at t = 320 ms, 2 ms
+    p.schedule(1, S2_TIME, 2)    # S2 at t = 320 ms, 2 ms
     s = myokit.SimulationOpenCL(m, p, ncells=(nx, ny), rl=True,
                                 precision=precision)
     s.set_step_size(0.01)
     s.set_conductance(9.5, 9.5)
     s.set_paced_cells(4, ny, 0, 0)
     return s
+
+
+def advance(s, seg, log_interval=2.0):
+    """One bench step: ``seg`` ms of the sheet from its current time. A
+    step never crosses 320 ms (the segments divide it); at 320 ms the
+    paced region becomes the lower-left quadrant for S2."""
+    if abs(s.time() - S2_TIME) < 1e-9:
+        s.set_paced_cells(s._nx // 2, s._ny // 2, 0, 0)
+    return s.run_block(seg, ['membrane.V'], log_interval=log_interval)
+
+
+def cpu_model():
+    """The host CPU's model name (``/proc/cpuinfo``), or None."""
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(nx, ny, sample_ms, threads):
@@ -335,7 +364,7 @@ def other_workload(args):
             cv, cs, cel = _cpu_sample(oracle, sample, threads)
             line['cpu_baseline'] = {
                 'value': cv, 'unit': 'cell-steps/s', 'cores': threads,
-                'kind': 'port',
+                'kind': 'port', 'cpu': cpu_model(),
                 'sample': 'CPU oracle (double-precision C, OpenMP) on the '
                           'first %g ms (%d time steps) of the same workload, '
                           '%.1f s' % (sample, cs, cel)}
@@ -468,11 +497,13 @@ def main():
     n = args.nx * args.ny
     steps_per_seg = int(round(seg / 0.01))
 
+    if abs(S2_TIME / seg - round(S2_TIME / seg)) > 1e-9:
+        raise ValueError('--segment-ms must divide 320 ms (the S2 time).')
     for _ in range(args.warmup):
-        s.run_block(seg, ['membrane.V'], log_interval=2.0)
+        advance(s, seg)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        out = s.run_block(seg, ['membrane.V'], log_interval=2.0)
+        out = advance(s, seg)
     el = time.perf_counter() - t0
     value = args.steps * steps_per_seg * n / el
     V = out['membrane.V']
@@ -481,7 +512,7 @@ def main():
     # Roofline leg: one more segment with every launch bracketed by events
     s.set_engine_option('time_kernels', 1)
     s.engine_timing(reset=True)
-    s.run_block(seg, ['membrane.V'], log_interval=2.0)
+    advance(s, seg)
     tm = s.engine_timing(reset=True)
     s.set_engine_option('time_kernels', 0)
     w = 4 if args.precision == 32 else 8
@@ -507,10 +538,13 @@ def main():
         'data': 'synthetic',
         'config': {
             'workload': '2D sheet %dx%d LR1991 (example.mmt), Rush-Larsen, '
-                        'dt=0.01 ms, gx=gy=9.5, S1 2 ms on left 4 columns, '
-                        'V of all cells logged every 2 ms; one step = %g ms '
-                        '(%d time steps)' % (args.nx, args.ny, seg,
-                                              steps_per_seg),
+                        'dt=0.01 ms, gx=gy=9.5, S1 2 ms on the left 4 '
+                        'columns at t=0, S2 2 ms on the lower-left quadrant '
+                        'at t=320 ms, V of all cells logged every 2 ms; one '
+                        'step = %g ms (%d time steps); timed steps cover '
+                        '%g-%g ms' % (args.nx, args.ny, seg, steps_per_seg,
+                                      args.warmup * seg,
+                                      (args.warmup + args.steps) * seg),
             'cells': n, 'time_steps_per_step': steps_per_seg,
             'precision': args.precision,
             'driver': s.engine_driver(),
@@ -574,11 +608,12 @@ def main():
                                        threads)
         line['cpu_baseline'] = {
             'value': cv, 'unit': 'cell-steps/s', 'cores': threads,
-            'kind': 'port',
+            'kind': 'port', 'cpu': cpu_model(),
             'sample': 'CPU oracle (double-precision C, OpenMP) on the first '
                       '%g ms (%d time steps) of the same %dx%d workload, '
-                      '%.1f s' % (args.cpu_sample_ms, csteps, args.nx,
-                                  args.ny, cel),
+                      '%.1f s; the reference SimulationOpenCL on a CPU '
+                      'OpenCL platform is unavailable (no PoCL)'
+                      % (args.cpu_sample_ms, csteps, args.nx, args.ny, cel),
         }
     print(json.dumps(line), flush=True)
 

@@ -152,9 +152,16 @@ int mkh_sim_set_pacing_list(void* sim, const long long* cells, long long n);
 /* Logging: colmap[q * n_cells + cell] is the log column that per-cell
  * quantity q of that cell is written to, or -1. n_cols is the number of
  * per-cell columns. nan_check enables the reference's NaN test on state 0
- * of cell 0 at every log point (openclsim.c:1079-1090). */
+ * of cell 0 (see mkh_sim_set_nan_cell) at every log point
+ * (openclsim.c:1079-1090). */
 int mkh_sim_set_logging(void* sim, const int* colmap, long long n_quant,
                         int n_cols, int nan_check);
+
+/* The cell whose state 0 the NaN test reads (default 0). The reference tests
+ * the caller's cell 0 (rvec_state[0], openclsim.c:1087-1089); a caller that
+ * holds its cells on the device in another order (e.g. a renumbered
+ * connection graph) passes the device index of its cell 0. */
+int mkh_sim_set_nan_cell(void* sim, long long cell);
 
 /* Begin a run from tmin to tmax with step dt (sim_init, openclsim.c:309).
  * events: n_events rows of (level, start, duration, period, multiplier),

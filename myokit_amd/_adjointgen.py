@@ -451,16 +451,15 @@ def generate_vjp_function(cm, precision, compute=None, islands=True):
 #: Single-precision multiply with 0 * x = 0 for every x, inf and NaN
 #: included (v_mul_legacy_f32): the adjoint guard in one instruction
 _VJP_LMUL = """
-/* The compiler inserts no wait states before inline assembly: the s_nop
- * gives the one that a VALU read of a transcendental's result (v_rcp_f32
- * just before, say) needs on gfx950; without it the multiply can read the
- * register's previous value. */
+/* The AMDGPU intrinsic behind v_mul_legacy_f32, declared by its LLVM name:
+ * the compiler schedules it like any VALU instruction and inserts the wait
+ * states its operands need (a read of a transcendental's result, say). */
+__device__ float mk_fmul_legacy(float, float)
+    __asm("llvm.amdgcn.fmul.legacy");
+
 __device__ __forceinline__ float mk_lmul(const float g, const float v)
 {
-    float r;
-    asm("s_nop 0\\n\\tv_mul_legacy_f32 %0, %1, %2"
-        : "=v"(r) : "v"(g), "v"(v));
-    return r;
+    return mk_fmul_legacy(g, v);
 }
 """
 

@@ -5,6 +5,7 @@ The library is built in-tree by ``myokit_amd/_build.py``. There is no
 fallback: if the library is missing, or no HIP device is present when a
 simulation is created, an error is raised.
 """
+import atexit
 import ctypes
 import os
 import threading
@@ -71,6 +72,7 @@ SIGNATURES = {
     'mkh_sim_set_pacing_rect': (_i, [_vp, _ll, _ll, _ll, _ll]),
     'mkh_sim_set_pacing_list': (_i, [_vp, _pll, _ll]),
     'mkh_sim_set_logging': (_i, [_vp, _pi, _ll, _i, _i]),
+    'mkh_sim_set_nan_cell': (_i, [_vp, _ll]),
     'mkh_sim_init': (_i, [_vp, _d, _d, _d, _pd, _ll, _d, _pll]),
     'mkh_sim_set_log': (_i, [_vp, _vp, _pd, _pd]),
     'mkh_sim_step': (_i, [_vp, _pd, _pi]),
@@ -100,6 +102,54 @@ SIGNATURES = {
 
 _lib = None
 
+# Set once the library's handles have been released at interpreter exit:
+# after that, finalizers that run late (engine objects collected during
+# interpreter teardown, page-locked blocks of arrays still referenced) make
+# no further calls into the library or the HIP runtime.
+closed = False
+
+# Live device handles, released by _shutdown: a weak reference to each
+# owner (an object with a handle ``h``) and the function that destroys the
+# handle (e.g. mkh_sim_destroy), keyed by id of the owner.
+_handles = {}
+_programs = []
+
+
+def track(owner, destroy):
+    """Registers ``owner.h`` to be released by ``destroy(h)`` at interpreter
+    exit, unless the owner is collected (and calls ``untrack``) first."""
+    _handles[id(owner)] = (weakref.ref(owner), destroy)
+
+
+def untrack(owner):
+    _handles.pop(id(owner), None)
+
+
+def track_program(p):
+    _programs.append(p)
+
+
+def _shutdown():
+    """Releases every engine, program and pooled page-locked block while the
+    HIP runtime (and any profiler attached to it) is still up. Registered
+    with atexit when the library is loaded, so it runs before the C
+    runtime's exit handlers."""
+    global closed
+    if _lib is None or closed:
+        return
+    for ref, destroy in list(_handles.values()):
+        owner = ref()
+        h = getattr(owner, 'h', None)
+        if h:
+            destroy(h)
+            owner.h = None
+    _handles.clear()
+    for p in _programs:
+        _lib.mkh_program_free(p)
+    del _programs[:]
+    pinned.release_free()
+    closed = True
+
 
 def lib():
     """Loads the library (once) and declares every signature."""
@@ -116,6 +166,9 @@ def lib():
             f.restype = res
             f.argtypes = args
         _lib = L
+        atexit.register(_shutdown)
+    if closed:
+        raise DeviceError('The device library was released at exit.')
     return _lib
 
 
@@ -197,12 +250,21 @@ class _PinnedPool:
         return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
 
     def _give_back(self, addr, size):
+        if closed:
+            return      # released with the process
         with self._lock:
             if self._kept + size <= self.keep:
                 self._free.append((size, addr))
                 self._kept += size
                 return
         if _lib is not None:
+            _lib.mkh_host_free(addr)
+
+    def release_free(self):
+        """Frees the pooled blocks that no array uses."""
+        with self._lock:
+            blocks, self._free, self._kept = self._free, [], 0
+        for size, addr in blocks:
             _lib.mkh_host_free(addr)
 
 

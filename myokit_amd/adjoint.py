@@ -53,6 +53,8 @@ class TissueAdjoint:
         self.probes = np.ascontiguousarray(p.reshape(-1), dtype=np.int64)
         if np.any(self.probes < 0) or np.any(self.probes >= sim._ntotal):
             raise ValueError('Probe cell out of range.')
+        if len(np.unique(self.probes)) != len(self.probes):
+            raise ValueError('Probe cells must be distinct.')
         self.checkpoint_every = int(checkpoint_every)
         if self.checkpoint_every < 1:
             raise ValueError('checkpoint_every must be at least 1.')

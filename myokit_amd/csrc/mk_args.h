@@ -42,9 +42,12 @@ struct MkArgsT {
     long long npad;             /* padded cell count (SoA stride) */
     long long nx, ny;           /* grid size */
     long long px0, px1, py0, py1;   /* paced rectangle [px0,px1) x [py0,py1) */
+    long long nan_cell;         /* cell whose state 0 the NaN test reads: the
+                                   caller's cell 0 (openclsim.c:1087-1089) in
+                                   the device's cell order */
     int pace_mode;              /* 0 all cells, 1 rectangle, 2 mask, 3 none */
     int ncols;                  /* per-cell log columns */
-    int nan_check;              /* test state 0 of cell 0 at log steps */
+    int nan_check;              /* test state 0 of nan_cell at log steps */
     int ndom;                   /* number of domains (block driver) */
     R gx, gy;                   /* homogeneous conductances */
 };

@@ -32,8 +32,9 @@
  *             for every step of the launch; tile edges are exchanged
  *             through L2 with a step counter per tile.
  *
- * Halting: at a log step, the thread of cell 0 records the log row in
- * halt[0] if state 0 is NaN (openclsim.c:1087-1089). The host ends launches
+ * Halting: at a log step, the thread of the caller's cell 0 (nan_cell in
+ * the device's cell order) records the log row in halt[0] if state 0 is NaN
+ * (openclsim.c:1087-1089). The host ends launches
  * at log steps, and a launch returns at once if halt[0] holds a row other
  * than its own, so steps after the halting step are never taken.
  */
@@ -186,7 +187,7 @@ mk_cells(const MkArgs a, const int s0, const int s1, Real* const traj)
     for (int s = s0; s < s1; ++s) {
         const MkStep p = a.steps[s];
         const Real pace = paced ? p.pace : (Real)0;
-        if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
+        if (p.row >= 0 && a.nan_check && c == a.nan_cell && y[0] != y[0])
             __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
         /* One instance of the cell code: logged values are register
          * copies, written out only at log steps */
@@ -248,7 +249,7 @@ mk_block(const MkArgs a, const int s0, const int s1, Real* const traj)
             const Real idiff = (Real)0;
 #endif
             const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace : (Real)0;
-            if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
+            if (p.row >= 0 && a.nan_check && c == a.nan_cell && y[0] != y[0])
                 __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
             /* One instance of the cell code: logged values are register
              * copies, written out only at log steps */
@@ -332,7 +333,7 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
         if (e != (Real)0) idiff += e;
     }
     const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace : (Real)0;
-    if (p.row >= 0 && a.nan_check && c == 0 && y[0] != y[0])
+    if (p.row >= 0 && a.nan_check && c == a.nan_cell && y[0] != y[0])
         __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
     /* One instance of the cell code: logged values are register
      * copies, written out only at log steps */
@@ -531,7 +532,8 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
                 }
                 const Real pace = paced[k] ? p.pace : (Real)0;
                 const long long c = ix + iy * nx;
-                if (p.row >= 0 && a.nan_check && c == 0 && y[k][0] != y[k][0])
+                if (p.row >= 0 && a.nan_check && c == a.nan_cell &&
+                    y[k][0] != y[k][0])
                     __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
                 /* One instance of the cell code: logged values are register
                  * copies, written out only at log steps */

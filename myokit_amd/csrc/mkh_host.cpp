@@ -224,6 +224,7 @@ struct SimBase {
     virtual int adj_reverse(int*, long long*) = 0;
     virtual int set_cell_order(const long long*, long long) = 0;
     virtual int copy_state_from(SimBase* src) = 0;
+    virtual int set_nan_cell(long long c) = 0;
 };
 
 template <typename R>
@@ -242,6 +243,7 @@ struct Sim : SimBase {
     R gx = 0, gy = 0;
     int pace_mode = 0;
     long long px0 = 0, px1 = 0, py0 = 0, py1 = 0;
+    long long nan_cell = 0;       /* device index of the caller's cell 0 */
     int ncols = 0, nan_check = 0;
     long long nquant = 0;
     int driver = MKH_DRIVER_GRID;
@@ -667,6 +669,13 @@ struct Sim : SimBase {
      * with the same logging skip the upload */
     std::vector<int> cmap_last;
 
+    int set_nan_cell(long long c) override
+    {
+        if (c < 0 || c >= n) return fail(MKH_E_ARG, "NaN cell out of range");
+        nan_cell = c;
+        return 0;
+    }
+
     int set_logging(const int* cmap, long long nq, int nc, int nanc) override
     {
         if (nq != cfg.n_quant)
@@ -783,6 +792,7 @@ struct Sim : SimBase {
         a.pace_mode = pace_mode;
         a.ncols = ncols;
         a.nan_check = nan_check;
+        a.nan_cell = nan_cell;
         a.ndom = (int)ndom;
         a.gx = gx;
         a.gy = gy;
@@ -1198,6 +1208,14 @@ struct Sim : SimBase {
         for (long long k = 0; k < nprobe; ++k)
             if (probes[k] < 0 || probes[k] >= n)
                 return fail(MKH_E_ARG, "probe cell out of range");
+        if (!trace_out) {
+            /* the loss keeps one probe index per cell (probe_of): a cell
+             * listed twice would lose the earlier probe's terms */
+            std::vector<long long> ps(probes, probes + nprobe);
+            std::sort(ps.begin(), ps.end());
+            if (std::adjacent_find(ps.begin(), ps.end()) != ps.end())
+                return fail(MKH_E_ARG, "probe cell listed twice");
+        }
         const size_t SN = (size_t)cfg.n_state * npad;
         if (trace_out) K = 1;
         const long long Kl = std::min<long long>(K, N);
@@ -2080,6 +2098,12 @@ int mkh_sim_set_logging(void* sim, const int* colmap, long long nq, int nc,
 {
     CHECK_SIM();
     return SIM->set_logging(colmap, nq, nc, nan_check);
+}
+
+int mkh_sim_set_nan_cell(void* sim, long long cell)
+{
+    CHECK_SIM();
+    return SIM->set_nan_cell(cell);
 }
 
 int mkh_sim_init(void* sim, double tmin, double tmax, double dt,

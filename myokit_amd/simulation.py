@@ -102,6 +102,7 @@ def _program(source, options, device=0):
             cache.encode() if cache else None, device, ctypes.byref(prog),
             log, 1 << 16), 'mkh_program_build')
         p = _PROGRAMS[key] = prog
+        _native.track_program(prog)
     return p
 
 
@@ -115,10 +116,12 @@ class _Engine:
             program, ctypes.byref(config), ctypes.byref(h)),
             'mkh_sim_create')
         self.h = h
+        _native.track(self, _native.lib().mkh_sim_destroy)
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and not _native.closed:
+                _native.untrack(self)
                 _native.lib().mkh_sim_destroy(self.h)
         except Exception:   # pragma: no cover
             pass
@@ -1161,7 +1164,12 @@ class SimulationOpenCL:
         self._configure(eng)
         L = _native.lib()
         h = eng.h
-        perm, _ = self._cell_order()
+        perm, iperm = self._cell_order()
+        # the NaN test reads the caller's cell 0 (openclsim.c:1087-1089)
+        nan_cell = 0 if iperm is None else int(iperm[0])
+        if getattr(eng, 'nan_cell', 0) != nan_cell:
+            _native.check(L.mkh_sim_set_nan_cell(h, nan_cell), 'nan_cell')
+            eng.nan_cell = nan_cell
         if perm is not None:
             # columns of the device's cells; cached per column map
             key = id(colmap)

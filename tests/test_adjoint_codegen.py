@@ -390,14 +390,14 @@ void run(const double* x, double* y, long n)
     assert np.isinf(y[-2]) and y[-1] == 0.0
 
 
-def test_inline_legacy_multiply_waits_for_its_operands(tmp_path):
-    # mk_lmul is inline assembly, before which the compiler inserts no wait
-    # states. On gfx950 a VALU read of a transcendental's result (v_rcp_f32,
-    # v_exp_f32) needs one; without it the multiply reads the register's
-    # previous value, deterministically for a given schedule. The reverse
+def test_legacy_multiply_is_compiler_visible(tmp_path):
+    # The adjoint guards (0 * x = 0 for every x) are v_mul_legacy_f32 through
+    # the llvm.amdgcn.fmul.legacy intrinsic, not inline assembly: the
+    # compiler then inserts the wait states their operands need (on gfx950
+    # a VALU read of a transcendental's result needs one). The reverse
     # kernels of a single-precision adjoint are compiled here (hiprtc needs
-    # no GPU) and every v_mul_legacy_f32 must follow an s_nop.
-    import re
+    # no GPU): they hold no inline assembly, and the guards reach the code
+    # object as legacy multiplies (or legacy multiply-adds).
     from myokit_amd import _native
     m = myokit.load_model('example')
     s = myokit.SimulationOpenCL(m, None, ncells=(64, 32), precision=32,
@@ -407,6 +407,7 @@ def test_inline_legacy_multiply_waits_for_its_operands(tmp_path):
                               [s._model.get('ina.gNa')],
                               [s._model.get('membrane.V')])
     src = _kernelgen.generate_kernel(cm, 32, 1, adjoint=True)
+    assert 'asm(' not in src.replace('__asm("llvm.amdgcn.fmul.legacy")', '')
     opts = _kernelgen.compile_options(32) + ' --save-temps'
     prog = ctypes.c_void_p()
     log = ctypes.create_string_buffer(1 << 16)
@@ -424,11 +425,8 @@ def test_inline_legacy_multiply_waits_for_its_operands(tmp_path):
     text = open(os.path.join(tmp_path, asm[0])).read()
     lines = [x.strip() for x in text.splitlines()
              if x.strip() and not x.strip().startswith((';', '.'))]
-    n = 0
-    for k, x in enumerate(lines):
-        if x.startswith('v_mul_legacy_f32'):
-            n += 1
-            assert lines[k - 1].startswith('s_nop'), lines[k - 2:k + 1]
+    n = sum(1 for x in lines if x.startswith(
+        ('v_mul_legacy_f32', 'v_mad_legacy_f32', 'v_fma_legacy_f32')))
     assert n >= 3 * 50      # three reverse kernels, dozens of guards each
 
 

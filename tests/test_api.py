@@ -292,3 +292,45 @@ def test_graph_cell_order(model):
     assert p2 is None or p2 is not perm
     s.set_conductance(9.5)
     assert s._cell_order() == (None, None)
+
+
+def test_adjoint_probe_checks(model):
+    # The loss keeps one probe per cell: a cell listed twice is an error
+    # rather than a silently dropped loss term (the library rejects it too)
+    from myokit_amd.adjoint import TissueAdjoint
+    s = myokit.SimulationOpenCL(model, ncells=10, precision=64)
+    TissueAdjoint(s, [0, 3, 9])
+    with pytest.raises(ValueError, match='distinct'):
+        TissueAdjoint(s, [0, 3, 3])
+    with pytest.raises(ValueError, match='out of range'):
+        TissueAdjoint(s, [0, 10])
+    s = myokit.SimulationOpenCL(model, ncells=(4, 3), precision=64)
+    with pytest.raises(ValueError, match='distinct'):
+        TissueAdjoint(s, [(1, 2), (1, 2)])
+
+
+def test_library_releases_handles_at_exit():
+    # The atexit hook frees every engine, program and pooled page-locked
+    # block before the C runtime's exit handlers run (and before a profiler
+    # attached to the HIP runtime shuts down); later finalizers make no
+    # library calls.
+    import subprocess
+    import sys
+    code = (
+        'import atexit, myokit_amd._native as n\n'
+        'n.lib()\n'
+        'class E: pass\n'
+        'e = E(); e.h = 12345\n'
+        'calls = []\n'
+        'n.track(e, calls.append)\n'
+        'n._shutdown()\n'
+        'assert calls == [12345] and e.h is None and n.closed\n'
+        'try:\n'
+        '    n.lib()\n'
+        'except n.DeviceError:\n'
+        '    print("closed")\n')
+    r = subprocess.run([sys.executable, '-c', code], capture_output=True,
+                       text=True, cwd=__import__('os').path.dirname(
+                           __import__('os').path.dirname(__file__)))
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == 'closed'

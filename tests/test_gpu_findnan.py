@@ -126,3 +126,51 @@ def test_manual_call(s2):
         assert np.isfinite(states[1]).all()
     finally:
         s2.set_protocol(protocol())
+
+
+def test_nan_in_user_cell_0_of_a_reordered_graph():
+    # The NaN test reads the caller's cell 0 (openclsim.c:1087-1089), also
+    # when the device holds a connection graph in another cell order. Here
+    # cell 0 and cell 1 form one component and cells 2..399 a random
+    # geometric graph; only cell 0 is paced, so only cells 0 and 1 go NaN
+    # (at the first step after 1.234 ms). The device cell at index 0 lies
+    # in the other component: a test of device cell 0 would never halt.
+    from oracle.cgen import OracleSimulation
+    rng = np.random.default_rng(7)
+    n = 400
+    pts = rng.uniform(size=(n, 2))
+    r2 = 6 / (np.pi * n)
+    conns = [(0, 1, 5.0)]
+    for i in range(2, n):
+        d2 = np.sum((pts[i + 1:] - pts[i]) ** 2, axis=1)
+        for j in np.nonzero(d2 < r2)[0]:
+            conns.append((i, i + 1 + int(j), float(rng.uniform(5, 12))))
+    s = myokit.SimulationOpenCL(clamped_model(), protocol(), ncells=n,
+                                precision=myokit.DOUBLE_PRECISION)
+    s.set_connections(conns)
+    s.set_paced_cell_list([0])
+    perm, iperm = s._cell_order()
+    assert perm is not None and perm[0] > 1
+    keys = ['engine.time'] + [str(c) + '.ica.Ca_i' for c in range(n)]
+    print('graph NaN: 400 cells,', len(conns), 'connections', flush=True)
+    d = s.run(5, log=keys, log_interval=0.1, report_nan=False)
+    o = OracleSimulation(clamped_model(), protocol(), ncells=n)
+    o.connections = conns
+    o.set_paced_cell_list([0])
+    o.dt = s._step_size
+    r = o.run(5, log_vars=['ica.Ca_i'], log_interval=0.1)
+    assert r['halted']
+    rows = len(d['engine.time'])
+    assert rows == r['rows'] and rows < 20
+    np.testing.assert_array_equal(np.asarray(d['engine.time']), r['time'])
+    V = np.stack([np.asarray(d[str(c) + '.ica.Ca_i']) for c in range(n)],
+                 axis=1)
+    np.testing.assert_array_equal(V, r['ica.Ca_i'])
+    assert np.isnan(V[-1, 0]) and np.isnan(V[-1, 1])
+    assert np.isfinite(V[:, 2:]).all()
+    time, icell, variable, value, states, bounds = s.find_nan(d)
+    assert time == pytest.approx(1.235, abs=1e-9)
+    assert icell == [0]
+    s.reset()
+    with pytest.raises(myokit.SimulationError, match='Time: +1.23'):
+        s.run(5, log=keys, log_interval=0.1)
