@@ -191,8 +191,15 @@ class CWriter:
         raise ValueError('Cannot write ' + t)
 
 
-def model_source(model, bound, rl_states, fields, quantities):
-    """C source of ``oracle_cell`` for a prepared model."""
+def model_source(model, bound, rl_states, fields, quantities,
+                 log_after_update=False):
+    """C source of ``oracle_cell`` for a prepared model.
+
+    Logged quantities are evaluated from the state at the step's time t,
+    before the update (``openclsim.cl:298-367`` writes ``inter_log`` from
+    the state it is about to advance). ``log_after_update`` plants a
+    mistake for the tests that show the golden data can catch it: the
+    quantities are then evaluated from the updated state."""
     from myokit_amd import _expressions as ex
     states = list(model.states())
     fields = list(fields)
@@ -244,10 +251,12 @@ def model_source(model, bound, rl_states, fields, quantities):
         else:
             out.append('    const double %s = %s;'
                        % (names[ex.Name(var)], w.w(var.rhs())))
-    out.append('    if (log) {')
+    log_block = ['    if (log) {']
     for j, var in enumerate(quantities):
-        out.append('        q[%d] = %s;' % (j, w.w(ex.Name(var))))
-    out.append('    }')
+        log_block.append('        q[%d] = %s;' % (j, w.w(ex.Name(var))))
+    log_block.append('    }')
+    if not log_after_update:
+        out.extend(log_block)
     for i, s in enumerate(states):
         if s in rl_states:
             inf, tau = rl_states[s]
@@ -256,6 +265,23 @@ def model_source(model, bound, rl_states, fields, quantities):
                        % (i, inf, inf, i, tau))
         else:
             out.append('    y[%d] = x%d + dt * dx%d;' % (i, i, i))
+    if log_after_update:
+        # the planted mistake: every logged name re-evaluated from the
+        # updated state (a nested scope shadows the step's values)
+        out.append('    {')
+        for i in range(len(states)):
+            out.append('    const double x%d = y[%d]; (void)x%d;' % (i, i, i))
+        for var in order:
+            if var.is_state():
+                i = states.index(var)
+                out.append('    const double dx%d = %s; (void)dx%d;'
+                           % (i, w.w(var.rhs()), i))
+            else:
+                out.append('    const double %s = %s; (void)%s;'
+                           % (names[ex.Name(var)], w.w(var.rhs()),
+                              names[ex.Name(var)]))
+        out.extend(log_block)
+        out.append('    }')
     out.append('}')
     return '\n'.join(out) + '\n'
 
@@ -349,6 +375,8 @@ class OracleSimulation:
         self.state = np.tile(init, self.n)
         self.threads = int(os.environ.get('OMP_NUM_THREADS', '0')) or \
             len(os.sched_getaffinity(0))
+        # planted mistake for the tests of the pins (see model_source)
+        self.log_after_update = False
 
     def set_paced_cells(self, nx=5, ny=5, x=0, y=0):
         # Normalisation of SimulationOpenCL.set_paced_cells
@@ -387,7 +415,8 @@ class OracleSimulation:
         quantities = [m.get(v) for v in log_vars]
         fields = list(self.fields.keys())
         src = model_source(m, self.bound, self.rl_states,
-                           [m.get(f) for f in fields], quantities)
+                           [m.get(f) for f in fields], quantities,
+                           log_after_update=self.log_after_update)
         lib = build(src)
         tmin = self.time
         tmax = tmin + duration

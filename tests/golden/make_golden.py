@@ -16,7 +16,12 @@ Each fixture pins something no closed form or reference test vector pins:
                        Simulation1d within 1e-13 in double precision
                        (test_simulation_opencl_vs_sim1d.py:118-136), so these
                        trajectories pin the 1D FE/RL stepping, coupling and
-                       pacing semantics.
+                       pacing semantics. Besides V they hold the logged
+                       diffusion current (membrane.i_diff), an intermediary
+                       (ica.ICa or isi.Isi) and engine.pace, which pin what
+                       a log row holds: values at the row's time, before the
+                       step from it. sim1d_br1977_reftest.npz is the
+                       configuration of that reference test itself.
 * pacing_py.json    -- myokit.PacingSystem sequences (time, next time, level)
                        for random event protocols: pins the event system.
 
@@ -92,35 +97,56 @@ def rl_forms():
 def sim1d():
     cases = [
         # name, model, ncells, npaced, g, rl, dt, duration, log_interval,
-        # protocol (level, start, duration, period, multiplier)
+        # protocol (level, start, duration, period, multiplier),
+        # intermediary logged besides V and i_diff, stimulus amplitude
+        # (None = the model's)
         ('sim1d_lr1991_fe', 'lr1991', 20, 5, 9.5, False, 0.005, 30.0, 0.5,
-         (1, 0, 2, 1000, 0)),
+         (1, 0, 2, 1000, 0), 'ica.ICa', None),
         ('sim1d_lr1991_rl', 'lr1991', 20, 5, 9.5, True, 0.01, 30.0, 0.5,
-         (1, 0, 2, 1000, 0)),
+         (1, 0, 2, 1000, 0), 'ica.ICa', None),
         ('sim1d_br1977_fe', 'br1977', 10, 2, 10.0, False, 0.005, 15.0, 0.5,
-         (1, 1, 2, 1000, 0)),
+         (1, 1, 2, 1000, 0), 'isi.Isi', None),
         ('sim1d_lr1991_cfg1', 'lr1991', 200, 5, 9.5, False, 0.005, 60.0, 1.0,
-         (1, 0, 2, 1000, 0)),
+         (1, 0, 2, 1000, 0), 'ica.ICa', None),
+        # test_simulation_opencl_vs_sim1d.py:28-57: BR1977 with the
+        # stimulus amplitude set to -80, blocktrain(1000, 2, offset=3),
+        # 10 cells, 2 paced, dt 0.005, 15 ms logged every 0.5 ms; Simulation1d
+        # uses its default conductance
+        ('sim1d_br1977_reftest', 'br1977', 10, 2, None, False, 0.005, 15.0,
+         0.5, (1, 3, 2, 1000, 0), 'isi.Isi', -80.0),
     ]
-    for (name, model, n, npaced, g, rl, dt, dur, li, ev) in cases:
+    for (name, model, n, npaced, g, rl, dt, dur, li, ev, inter,
+         amp) in cases:
         m = myokit.load_model(MODELS[model])
+        if amp is not None:
+            m.get('membrane.i_stim.amplitude').set_rhs(amp)
         p = myokit.Protocol()
         p.schedule(*ev)
         s = myokit.Simulation1d(m, p, ncells=n, rl=rl)
+        if g is None:
+            g = s.conductance()
         s.set_conductance(g)
         s.set_paced_cells(npaced)
         s.set_step_size(dt)
         vname = m.label('membrane_potential').qname()
+        dname = m.binding('diffusion_current').qname()
+        qs = [vname, dname, inter]
         keys = ['engine.time', 'engine.pace'] + [
-            str(i) + '.' + vname for i in range(n)]
+            str(i) + '.' + q for q in qs for i in range(n)]
         d = s.run(dur, log=keys, log_interval=li).npview()
-        V = np.stack([d[str(i) + '.' + vname] for i in range(n)], axis=1)
+        cols = {}
+        for key, q in zip(['V', 'i_diff', 'inter'], qs):
+            cols[key] = np.stack([d[str(i) + '.' + q] for i in range(n)],
+                                 axis=1)
         np.savez(os.path.join(HERE, name + '.npz'),
-                 time=d['engine.time'], pace=d['engine.pace'], V=V,
+                 time=d['engine.time'], pace=d['engine.pace'],
                  final_state=np.array(s.state()),
                  config=np.array([n, npaced, g, float(rl), dt, dur, li]),
-                 event=np.array(ev, dtype=float))
-        print(name, V.shape)
+                 event=np.array(ev, dtype=float),
+                 inter_name=np.array(inter),
+                 amplitude=np.array(np.nan if amp is None else amp),
+                 **cols)
+        print(name, cols['V'].shape)
 
 
 def pacing():

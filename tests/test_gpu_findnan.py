@@ -151,23 +151,32 @@ def test_nan_in_user_cell_0_of_a_reordered_graph():
     s.set_paced_cell_list([0])
     perm, iperm = s._cell_order()
     assert perm is not None and perm[0] > 1
-    keys = ['engine.time'] + [str(c) + '.ica.Ca_i' for c in range(n)]
+    # State 0 of the clamped model (the one the NaN test reads) is ina.m;
+    # the diffusing "membrane potential" is Ca_i
+    qs = ['ina.m', 'ica.Ca_i']
+    keys = ['engine.time'] + [str(c) + '.' + q for q in qs for c in range(n)]
     print('graph NaN: 400 cells,', len(conns), 'connections', flush=True)
     d = s.run(5, log=keys, log_interval=0.1, report_nan=False)
     o = OracleSimulation(clamped_model(), protocol(), ncells=n)
     o.connections = conns
     o.set_paced_cell_list([0])
     o.dt = s._step_size
-    r = o.run(5, log_vars=['ica.Ca_i'], log_interval=0.1)
+    r = o.run(5, log_vars=qs, log_interval=0.1)
     assert r['halted']
     rows = len(d['engine.time'])
     assert rows == r['rows'] and rows < 20
     np.testing.assert_array_equal(np.asarray(d['engine.time']), r['time'])
-    V = np.stack([np.asarray(d[str(c) + '.ica.Ca_i']) for c in range(n)],
-                 axis=1)
-    np.testing.assert_array_equal(V, r['ica.Ca_i'])
-    assert np.isnan(V[-1, 0]) and np.isnan(V[-1, 1])
-    assert np.isfinite(V[:, 2:]).all()
+    for q in qs:
+        V = np.stack([np.asarray(d[str(c) + '.' + q]) for c in range(n)],
+                     axis=1)
+        np.testing.assert_array_equal(V, r[q])
+        # only the paced cell 0 and its partner cell 1 can go NaN
+        assert np.isfinite(V[:, 2:]).all()
+        assert np.isfinite(V[:-1]).all()
+    # the run halts at the first log row where the caller's cell 0 has a
+    # NaN state 0, and at no earlier row
+    m0 = np.asarray(d['0.ina.m'])
+    assert np.isnan(m0[-1]) and np.isfinite(m0[:-1]).all()
     time, icell, variable, value, states, bounds = s.find_nan(d)
     assert time == pytest.approx(1.235, abs=1e-9)
     assert icell == [0]

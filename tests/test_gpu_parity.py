@@ -357,3 +357,28 @@ def test_fp64_graph_cell_order_boundaries():
     V = np.stack([np.asarray(d[k]) for k in keys], axis=1)
     np.testing.assert_array_equal(V, r['membrane.V'])
     np.testing.assert_array_equal(np.array(s.state()), o.state)
+
+
+@pytest.mark.parametrize('driver', ['grid', 'sheet'])
+def test_fp64_transposed_sheet(driver):
+    # The (nx, ny) sheet with (gx, gy) and a corner stimulus equals the
+    # transposed sheet with (gy, gx), bit for bit, on the device too (the
+    # CPU test of the same property pins the oracle's y term)
+    nx, ny = 80, 64
+    out = []
+    for shape, g, paced in (((nx, ny), (9.5, 4.0), (6, 5, 0, 0)),
+                            ((ny, nx), (4.0, 9.5), (5, 6, 0, 0))):
+        s = myokit.SimulationOpenCL(
+            myokit.load_model('example'), myokit.pacing.blocktrain(1000, 2),
+            ncells=shape, rl=True, precision=myokit.DOUBLE_PRECISION)
+        s.set_tuning(driver=driver)
+        s.set_step_size(0.01)
+        s.set_conductance(*g)
+        s.set_paced_cells(*paced)
+        s.run(10.0, log=[], log_interval=1.0)
+        assert s.engine_driver() == driver
+        out.append(np.array(s.state()).reshape(shape[1], shape[0], -1))
+    np.testing.assert_array_equal(out[0], out[1].transpose(1, 0, 2))
+    V = out[0][:, :, 0]
+    assert np.max(V[0, :]) - np.min(V[0, :]) > 10
+    assert np.max(V[:, 0]) - np.min(V[:, 0]) > 10
