@@ -590,15 +590,25 @@ def _island(r, e, k, tab):
 # 0.1 per mV): 3e-7 at the window's edge (|a x + m| = 1.5, u = 15 mV).
 ISLAND_WINDOW = 1.5
 
+# Half-width, in units of the variable, of the window around the root of a
+# cancelling factor a x + m (ik.xi's V + 77 below its numerator
+# exp(0.04 u) - 1): outside it the single-precision quotient rule's two
+# terms, each ~ 1/u, cancel to ~0.02 with a relative error of about
+# 3e-6 / u, under 1e-6 at 5 mV; its share of mu is at the 1e-8 level there
+# (tests/test_adjoint_codegen.py). The resting potential (-84 mV) and the
+# plateau lie outside, so most cells skip the island.
+ISLAND_WINDOW_LINEAR = 5.0
+
 
 def _island_window(r, e):
     """C condition (single precision) under which the island of ``e`` must
     be computed in double precision, or None if it always must: every
     cancelling factor of every denominator in ``e`` has the form 1 -
-    exp(a x + m) or exp(a x + m) - 1 with x one variable and a, m
-    literals, and the condition is |a x + m| < ISLAND_WINDOW for one of
-    them. Quotients whose cancelling factor is anything else (ik.xi's
-    V + 77) keep the island everywhere."""
+    exp(a x + m) or exp(a x + m) - 1, the condition then being
+    |a x + m| < ISLAND_WINDOW, or the form a x + m, the condition then
+    being |a x + m| < ISLAND_WINDOW_LINEAR |a|, with x one variable and a,
+    m literals. Quotients with any other cancelling factor keep the island
+    everywhere."""
     conds = []
 
     def factors(d):
@@ -624,12 +634,17 @@ def _island_window(r, e):
                     continue
                 arg = expm1_arg(f)
                 aff = r.w.affine(arg) if arg is not None else None
-                if aff is None:
-                    return False
+                if aff is not None:
+                    width = ISLAND_WINDOW
+                else:
+                    aff = r.w.affine(f)
+                    if aff is None or aff[1] == 0:
+                        return False
+                    width = ISLAND_WINDOW_LINEAR * abs(aff[1])
                 conds.append('__builtin_fabsf(__builtin_fmaf(' + aff[0]
                              + ', ' + real_literal(aff[1], r.p) + ', '
                              + real_literal(aff[2], r.p) + ')) < '
-                             + real_literal(ISLAND_WINDOW, r.p))
+                             + real_literal(width, r.p))
         return all(walk(y) for y in x)
     if r.p == DOUBLE_PRECISION or not walk(e) or not conds:
         return None

@@ -191,15 +191,10 @@ class CWriter:
         raise ValueError('Cannot write ' + t)
 
 
-def model_source(model, bound, rl_states, fields, quantities,
-                 log_after_update=False):
-    """C source of ``oracle_cell`` for a prepared model.
-
-    Logged quantities are evaluated from the state at the step's time t,
-    before the update (``openclsim.cl:298-367`` writes ``inter_log`` from
-    the state it is about to advance). ``log_after_update`` plants a
-    mistake for the tests that show the golden data can catch it: the
-    quantities are then evaluated from the updated state."""
+def model_c_parts(model, bound, fields):
+    """What both C writers of a model need: (writer, states, per-cell
+    constants, solvable order, names). Constants that depend on no field
+    are folded to numbers; those that do are evaluated per cell (``pc``)."""
     from myokit_amd import _expressions as ex
     states = list(model.states())
     fields = list(fields)
@@ -230,7 +225,20 @@ def model_source(model, bound, rl_states, fields, quantities,
         if not var.is_state():
             names[ex.Name(var)] = 'v' + str(k)
             k += 1
-    w = CWriter(names, constants)
+    return CWriter(names, constants), states, percell, order, names
+
+
+def model_source(model, bound, rl_states, fields, quantities,
+                 log_after_update=False):
+    """C source of ``oracle_cell`` for a prepared model.
+
+    Logged quantities are evaluated from the state at the step's time t,
+    before the update (``openclsim.cl:298-367`` writes ``inter_log`` from
+    the state it is about to advance). ``log_after_update`` plants a
+    mistake for the tests that show the golden data can catch it: the
+    quantities are then evaluated from the updated state."""
+    from myokit_amd import _expressions as ex
+    w, states, percell, order, names = model_c_parts(model, bound, fields)
     out = ['#include <math.h>', '#include "mk_math.h"',
            'const int oracle_n_state = %d;' % len(states),
            'const int oracle_n_field = %d;' % len(fields),

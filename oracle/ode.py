@@ -1,0 +1,229 @@
+"""
+CPU ORACLE -- test infrastructure, not part of the product.
+
+Only ``tests/`` may import this module, and only as the checker.
+
+Single-cell reference solutions standing in for myokit's CVODES simulation
+(``myokit.Simulation``, ``myokit/_sim/cvodessim.py``, with forward
+sensitivities from ``cvodessim.py:20-164`` and ``cmodel.py:244-277``).
+SUNDIALS is not installed here, so CVODES itself cannot run; its published
+algorithm is restated with scipy:
+
+* the model's right-hand side is written as plain C (``ode_rhs``: every
+  equation in dependency order, the same writer and portable double
+  precision math as the tissue oracle, ``oracle/cgen.py``) and called
+  through ctypes;
+* ``solve_ivp`` (Radau, rtol = atol = 1e-10, the substitute the scope table
+  names, tighter than the 1e-8 the reference's comparison tests set) runs
+  from one pacing discontinuity to the next, as CVODES is restarted at
+  every event of the protocol (``cvodessim.c``), with the pacing level
+  constant in each piece (``myokit_amd.PacingSystem``, pinned separately
+  against the reference's PacingSystem);
+* log points are ``tmin + k * log_interval`` inside [tmin, tmax), each
+  holding the state, the pacing level and the logged intermediaries at that
+  time (periodic logging of ``cvodessim.c``);
+* forward sensitivities integrate ds/dt = (df/dy) s + df/dp beside the
+  state, with the right-hand side of the sensitivity equations formed by
+  centred directional difference quotients of f, which is what CVODES does
+  when no sensitivity function is given (its default, CV_CENTERED).
+
+Pins (``tests/test_ode_substitute.py``): forward Euler from the tissue
+oracle converges to these solutions at first order as dt shrinks, and the
+sensitivities equal central differences of whole runs with perturbed
+parameters.
+"""
+import ctypes
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+
+from oracle.cgen import BUILD, MATH_H, model_c_parts
+
+RTOL = ATOL = 1e-10
+
+
+def rhs_source(model, bound, fields, quantities):
+    """C source of ``ode_rhs(t, y, pace, fv, dy, q, log)`` for a prepared
+    model: the derivatives and, if ``log``, the quantities."""
+    from myokit_amd import _expressions as ex
+    w, states, percell, order, names = model_c_parts(model, bound, fields)
+    out = ['#include <math.h>', '#include "mk_math.h"',
+           'void ode_rhs(double t, const double* y, double pace,'
+           ' const double* fv, double* dy, double* q, int log)',
+           '{',
+           '    const double idiff = 0; (void)idiff; (void)t; (void)pace;'
+           ' (void)fv;']
+    for i in range(len(states)):
+        out.append('    const double x%d = y[%d];' % (i, i))
+    for i, var in enumerate(percell):
+        out.append('    const double pc%d = %s;' % (i, w.w(var.rhs())))
+    for var in order:
+        if var.is_state():
+            i = states.index(var)
+            out.append('    const double dx%d = %s;' % (i, w.w(var.rhs())))
+            out.append('    dy[%d] = dx%d;' % (i, i))
+        else:
+            out.append('    const double %s = %s;'
+                       % (names[ex.Name(var)], w.w(var.rhs())))
+    out.append('    if (log) {')
+    for j, var in enumerate(quantities):
+        out.append('        q[%d] = %s;' % (j, w.w(ex.Name(var))))
+    out.append('    }')
+    out.append('}')
+    return '\n'.join(out) + '\n'
+
+
+def _build(source):
+    os.makedirs(BUILD, exist_ok=True)
+    with open(MATH_H) as f:
+        mh = f.read()
+    h = hashlib.sha1((source + mh).encode()).hexdigest()[:16]
+    so = os.path.join(BUILD, 'ode_' + h + '.so')
+    if not os.path.isfile(so):
+        src = os.path.join(BUILD, 'ode_' + h + '.c')
+        with open(src, 'w') as f:
+            f.write(source)
+        cmd = ['gcc', '-O2', '-fPIC', '-shared', '-ffp-contract=off',
+               '-fno-fast-math', '-I', os.path.dirname(MATH_H), '-o',
+               so + '.tmp', src, '-lm']
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError('ODE build failed:\n' + r.stderr)
+        os.replace(so + '.tmp', so)
+    lib = ctypes.CDLL(so)
+    d = ctypes.POINTER(ctypes.c_double)
+    lib.ode_rhs.restype = None
+    lib.ode_rhs.argtypes = [ctypes.c_double, d, ctypes.c_double, d, d, d,
+                            ctypes.c_int]
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OdeCell:
+    """A single cell of ``model`` (unprepared myokit_amd Model) paced by
+    ``protocol``; ``fields`` maps variable names to values that replace
+    their constants (for the sensitivity parameters)."""
+
+    def __init__(self, model, protocol=None, fields=None):
+        from myokit_amd.simulation import _prepare_model
+        self.model, _, _, self.bound = _prepare_model(model, False, False)
+        self.protocol = protocol
+        self.fields = dict(fields or {})
+        self.state = np.array(self.model.initial_values(True), dtype=float)
+        self.time = 0.0
+        self._libs = {}
+
+    def _lib(self, quantities):
+        key = tuple(quantities)
+        if key not in self._libs:
+            m = self.model
+            src = rhs_source(m, self.bound, [m.get(f) for f in self.fields],
+                             [m.get(q) for q in quantities])
+            self._libs[key] = _build(src)
+        return self._libs[key]
+
+    def _pieces(self, tmin, tmax):
+        """[(t0, t1, pace)] covering [tmin, tmax), pace constant in each."""
+        from myokit_amd import PacingSystem
+        ps = PacingSystem(self.protocol, tmin)
+        out = []
+        t = tmin
+        while t < tmax:
+            level = ps.pace()
+            tn = ps.next_time()
+            t1 = min(tn, tmax) if tn > t else tmax
+            out.append((t, t1, level))
+            ps.advance(t1)
+            t = t1
+        return out
+
+    def _f(self, lib, fv):
+        n = len(self.state)
+        dy = np.zeros(n)
+        q = np.zeros(1)
+        fvp = _dp(fv)
+
+        def f(t, y, pace):
+            y = np.ascontiguousarray(y, dtype=float)
+            lib.ode_rhs(t, _dp(y), pace, fvp, _dp(dy), _dp(q), 0)
+            return dy.copy()
+        return f
+
+    def run(self, duration, log_vars=('membrane.V',), log_interval=1.0,
+            sensitivities=()):
+        """Runs ``duration`` from the current state and time. Returns a dict
+        with ``time``, ``pace``, per logged variable an array of values at
+        the log points, and, for each name in ``sensitivities`` (a field),
+        ``('dV', name)``: d(state 0 ... n)/d(field) at the log points as an
+        array (rows, states)."""
+        from scipy.integrate import solve_ivp
+        lv = list(log_vars)
+        lib = self._lib(lv)
+        n = len(self.state)
+        fnames = list(self.fields)
+        fv = np.array([self.fields[k] for k in fnames] or [0.0], dtype=float)
+        sens = [fnames.index(s) for s in sensitivities]
+        ns = len(sens)
+        f = self._f(lib, fv)
+        tmin, tmax = self.time, self.time + duration
+        nlog = 0
+        while tmin + nlog * log_interval < tmax:
+            nlog += 1
+        tlog = tmin + np.arange(nlog) * log_interval
+        hs = [1e-7 * max(1.0, abs(fv[k])) for k in sens]
+
+        def rhs(t, z, pace):
+            y = z[:n]
+            dz = np.empty_like(z)
+            dz[:n] = f(t, y, pace)
+            for j, k in enumerate(sens):
+                # centred directional difference quotient of f along
+                # (s_j, e_k): (df/dy) s_j + df/dp_k
+                s = z[n + j * n:n + (j + 1) * n]
+                h = hs[j]
+                fv[k] += h
+                a = f(t, y + h * s, pace)
+                fv[k] -= 2 * h
+                b = f(t, y - h * s, pace)
+                fv[k] += h
+                dz[n + j * n:n + (j + 1) * n] = (a - b) / (2 * h)
+            return dz
+
+        z = np.concatenate([self.state, np.zeros(n * ns)])
+        Z = np.zeros((nlog, len(z)))
+        P = np.zeros(nlog)
+        for t0, t1, pace in self._pieces(tmin, tmax):
+            sel = np.nonzero((tlog >= t0) & (tlog < t1))[0]
+            # the log points of the piece come from the solver's dense
+            # output (as CVODES interpolates to log points), the state at
+            # the piece's end from its last step
+            r = solve_ivp(rhs, (t0, t1), z, method='Radau', rtol=RTOL,
+                          atol=ATOL, args=(pace,),
+                          t_eval=np.concatenate([tlog[sel], [t1]]),
+                          first_step=min(1e-3, t1 - t0))
+            if not r.success:
+                raise RuntimeError('solve_ivp failed: ' + r.message)
+            Z[sel] = r.y[:, :-1].T
+            P[sel] = pace
+            z = r.y[:, -1].copy()
+        out = {'time': tlog, 'pace': P}
+        # logged quantities from the state at each log point
+        q = np.zeros(max(1, len(lv)))
+        dy = np.zeros(n)
+        vals = np.zeros((nlog, len(lv)))
+        for i in range(nlog):
+            y = np.ascontiguousarray(Z[i, :n])
+            lib.ode_rhs(tlog[i], _dp(y), P[i], _dp(fv), _dp(dy), _dp(q), 1)
+            vals[i] = q[:len(lv)]
+        for j, v in enumerate(lv):
+            out[v] = vals[:, j].copy()
+        for j, name in enumerate(sensitivities):
+            out[('dV', name)] = Z[:, n + j * n:n + (j + 1) * n].copy()
+        self.state = z[:n].copy()
+        self.time = tmax
+        return out

@@ -452,8 +452,8 @@ def test_island_window_edges(tmp_path):
     # ina.m's alpha, 0.32 (V + 47.13) / (1 - exp(-0.1 (V + 47.13))), is
     # differentiated in double precision only within 15 mV of its singular
     # point (|-0.1 (V + 47.13)| < 1.5) and by the single-precision rule
-    # outside; ik.xi, whose cancelling factor is V + 77, keeps its island
-    # everywhere. On both sides of the window's edges and far from the
+    # outside; ik.xi, whose cancelling factor is V + 77, within 5 mV of -77
+    # mV. On both sides of the windows' edges and far from the
     # singular point, d m' / dV of the single-precision VJP agrees with the
     # double one to 1e-5 relative. Within 0.3 mV of the point the single-
     # precision forward values themselves (1 - exp(-0.1 u) for small u) carry
@@ -464,8 +464,9 @@ def test_island_window_edges(tmp_path):
     model = myokit.load_model('example')
     cell = Cell(tmp_path, model, True, ['ina.gNa'])
     src = generate_vjp_function(cell.cm, 32)
-    assert src.count('if (__builtin_fabsf(__builtin_fmaf(') == 1
+    assert src.count('if (__builtin_fabsf(__builtin_fmaf(') == 2
     assert _adjointgen.ISLAND_WINDOW == 1.5
+    assert _adjointgen.ISLAND_WINDOW_LINEAR == 5.0
     fv = np.array([16.0])
     names = [s.qname() for s in cell.cm.states]
     iv, im = names.index('membrane.V'), names.index('ina.m')
@@ -481,3 +482,19 @@ def test_island_window_edges(tmp_path):
                        0.0, 0.0, fv, lam)[0]
         tol = 1e-4 if abs(dv) <= 0.3 else 1e-5
         assert m32[iv] == pytest.approx(m64[iv], rel=tol), dv
+    # ik.xi's window: the partial d V' / dV with ik.x open (IK at full
+    # weight), across the edges at -77 +- 5 mV, at rest (-84 mV) and on the
+    # plateau
+    ix = names.index('ik.x')
+    lam = np.zeros(cell.ns)
+    lam[iv] = 1.0
+    for dv in (-40, -7, -5.2, -5.0, -4.8, -1, -0.01, 0.01, 1, 4.8, 5.0, 5.2,
+               7, 80):
+        y = cell.y0.copy()
+        y[iv] = -77.0 + dv
+        y[ix] = 1.0
+        y = y.astype(np.float32).astype(float)
+        m32 = cell.vjp(32, y, 1.0, 0.01, 0.0, 0.0, fv, lam)[0]
+        m64 = cell.vjp(64, y, float(np.float32(1.0)), float(np.float32(0.01)),
+                       0.0, 0.0, fv, lam)[0]
+        assert m32[iv] == pytest.approx(m64[iv], rel=1e-5), dv
