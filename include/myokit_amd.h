@@ -264,6 +264,25 @@ int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,
                   const long long* probes, long long n_probes, double* trace,
                   long long n_rows);
 
+/* mkh_adj_trace without any device allocation. No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, result and errors as mkh_adj_trace; the step table, the state
+ * copies the run advances, the probe list and the device trace live in a
+ * static block of MKH_STATIC_SCRATCH_BYTES per device (held until the
+ * process ends) instead of the simulation's work buffers. A run whose
+ * buffers exceed it (about 2 * n_cells * n_state * precision/8 bytes for
+ * the states, plus 16-32 bytes per time step and 8 per probe and step)
+ * fails with MKH_E_ARG before anything is launched. Every call on a device
+ * uses the same block: calls that overlap on one device (other threads,
+ * other simulations) race and must be serialised by the caller. The call
+ * allocates no device or page-locked memory and frees none. */
+#define MKH_STATIC_SCRATCH_BYTES (32LL << 20)
+int mkh_adj_trace_static(void* sim, double tmin, double tmax, double dt,
+                         const double* events, long long n_events,
+                         const long long* probes, long long n_probes,
+                         double* trace, long long n_rows);
+
 /* The execution driver (MKH_DRIVER_*) the simulation currently uses. */
 int mkh_sim_driver(void* sim, int* driver);
 

@@ -95,6 +95,8 @@ SIGNATURES = {
     'mkh_host_alloc': (_i, [_ll, ctypes.POINTER(ctypes.c_void_p)]),
     'mkh_host_free': (_i, [_vp]),
     'mkh_adj_trace': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd, _ll]),
+    'mkh_adj_trace_static': (_i, [_vp, _d, _d, _d, _pd, _ll, _pll, _ll, _pd,
+                                  _ll]),
     'mkh_pacing_levels': (_i, [_pd, _ll, _d, _pd, _ll, _pd, _pd]),
     'mkh_plan': (_i, [_d, _d, _d, _pd, _ll, _d, _pd, _pd, _pd, _pi, _ll,
                       _pll]),

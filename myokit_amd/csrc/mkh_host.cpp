@@ -35,6 +35,15 @@
 #include "mk_args.h"
 #include "mk_esys.h"
 
+#include <mutex>
+
+/* Device scratch of mkh_adj_trace_static, the entry point that allocates
+ * nothing: one block per device, held from the library's load until the
+ * process ends. On the host this name stands for the block but is not its
+ * device address: host code uses only the address scratch_address() looks
+ * up (per device, with that device current). */
+static __device__ __align__(256) char scratch[MKH_STATIC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -164,6 +173,34 @@ int pacing_error(int f)
     return fail(MKH_E_PACING_INVALID, "E-Pacing error: invalid protocol.");
 }
 
+/* Device address of `scratch` on each device, looked up on the first
+ * call for that device (hipGetSymbolAddress with the device current) and
+ * kept: each device holds its own copy of the block at its own address. */
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[256] = {};
+
+int scratch_address(int device, void** ptr)
+{
+    if (device < 0 || device >= 256)
+        return fail(MKH_E_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int old = 0;
+        MKH_HIP(hipGetDevice(&old));
+        MKH_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(old);
+        if (e != hipSuccess || !p)
+            return fail(MKH_E_HIP, std::string("hipGetSymbolAddress(scratch) "
+                                               "failed: ") +
+                                       hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return 0;
+}
+
 /* -------------------------------------------------------------------------
  * Simulation
  * ---------------------------------------------------------------------- */
@@ -225,6 +262,9 @@ struct SimBase {
     virtual int set_cell_order(const long long*, long long) = 0;
     virtual int copy_state_from(SimBase* src) = 0;
     virtual int set_nan_cell(long long c) = 0;
+    virtual int trace_static(double, double, double, const double*,
+                             long long, const long long*, long long, double*,
+                             long long) = 0;
 };
 
 template <typename R>
@@ -1171,6 +1211,9 @@ struct Sim : SimBase {
     {
         /* trace_out != NULL: forward only, V at the probes after each step
          * into trace_out[s * nprobe + p] (nrows = its row capacity) */
+        if (trace_out)
+            return trace_run(tmin, tmax, dt, events, nev, probes, nprobe,
+                             trace_out, nrows, nullptr, 0);
         if (!prog->f_adj_step)
             return fail(MKH_E_STATE, "kernel was generated without the "
                                      "adjoint");
@@ -1202,13 +1245,12 @@ struct Sim : SimBase {
         if (N == 0) return fail(MKH_E_ARG, "empty run");
         if (N > INT_MAX) return fail(MKH_E_ARG, "run too long");
         if (nrows < N)
-            return fail(MKH_E_ARG, std::string(trace_out ? "trace" : "target")
-                                       + " needs one row per time step ("
+            return fail(MKH_E_ARG, "target needs one row per time step ("
                                        + std::to_string(N) + ")");
         for (long long k = 0; k < nprobe; ++k)
             if (probes[k] < 0 || probes[k] >= n)
                 return fail(MKH_E_ARG, "probe cell out of range");
-        if (!trace_out) {
+        {
             /* the loss keeps one probe index per cell (probe_of): a cell
              * listed twice would lose the earlier probe's terms */
             std::vector<long long> ps(probes, probes + nprobe);
@@ -1217,7 +1259,6 @@ struct Sim : SimBase {
                 return fail(MKH_E_ARG, "probe cell listed twice");
         }
         const size_t SN = (size_t)cfg.n_state * npad;
-        if (trace_out) K = 1;
         const long long Kl = std::min<long long>(K, N);
         const long long nseg = (N + Kl - 1) / Kl;
         const unsigned bt = (unsigned)cfg.block_threads;
@@ -1241,16 +1282,14 @@ struct Sim : SimBase {
         DevBuf<double>& gya = adj.gy;
         DevBuf<double>& lossa = adj.loss;
         DevBuf<double>& dtarget = adj.target;
-        DevBuf<double>& dtrace = adj.trace;
         DevBuf<int>& probe_of = adj.probe_of;
         DevBuf<long long>& dprobes = adj.probes;
         MKH_HIP(dsteps.alloc(N));
         MKH_HIP(hipMemcpy(dsteps.p, all.data(), N * sizeof(Step),
                           hipMemcpyHostToDevice));
-        /* trajectory: the longest segment plus its initial state (trace
-         * mode: two work buffers) */
-        const long long tslots = trace_out ? 2 : Kl + 1;
-        if (!trace_out && nseg > 1) MKH_HIP(ckpt.alloc((nseg - 1) * SN));
+        /* trajectory: the longest segment plus its initial state */
+        const long long tslots = Kl + 1;
+        if (nseg > 1) MKH_HIP(ckpt.alloc((nseg - 1) * SN));
         MKH_HIP(traj.alloc((size_t)tslots * SN));
         MKH_HIP(dprobes.alloc(std::max<long long>(1, nprobe)));
         if (nprobe > 0)
@@ -1261,10 +1300,7 @@ struct Sim : SimBase {
         j.probes = dprobes.p;
         j.n_probe = (int)nprobe;
         j.target_row = -1;
-        if (trace_out) {
-            MKH_HIP(dtrace.alloc(std::max<long long>(1, N * nprobe)));
-            j.trace = dtrace.p;
-        } else {
+        {
             MKH_HIP(muA.alloc(SN));
             MKH_HIP(muB.alloc(SN));
             MKH_HIP(abA.alloc(npad));
@@ -1315,14 +1351,14 @@ struct Sim : SimBase {
         const bool use_pers = driver == MKH_DRIVER_SHEET ||
                               driver == MKH_DRIVER_BLOCK ||
                               driver == MKH_DRIVER_CELLS;
-        const bool block_rev = driver == MKH_DRIVER_BLOCK && !trace_out;
+        const bool block_rev = driver == MKH_DRIVER_BLOCK;
         /* The persistent reverse sheet: 64 x 4 MK_ACPT tiles, all
          * co-resident; they are no more than the forward sheet's (4 *
          * sheet_cpt rows), whose edge buffer they use */
         const int rtx = sh_tx;
         const int rty = (int)((cfg.ny + 4 * MK_ACPT - 1) / (4 * MK_ACPT));
         bool sheet_rev = false;
-        if (driver == MKH_DRIVER_SHEET && !trace_out && prog->f_adj_sheet &&
+        if (driver == MKH_DRIVER_SHEET && prog->f_adj_sheet &&
             sh_edges.p && (size_t)rtx * rty * 4 * 64 * 2 <= sh_edges.n &&
             (size_t)rtx * rty <= nparts) {
             int per_cu = 0;
@@ -1387,34 +1423,6 @@ struct Sim : SimBase {
         R* wb = slot(1);
         MKH_HIP(hipMemcpyAsync(wa, state[cur].p, SN * sizeof(R),
                                hipMemcpyDeviceToDevice, stream));
-        if (trace_out) {
-            /* one step per launch, then gather the probes */
-            const unsigned pgrid = (unsigned)((nprobe + 63) / 64);
-            if (use_pers)
-                MKH_HIP(hipMemcpyAsync(wcur, state[cur].p, SN * sizeof(R),
-                                       hipMemcpyDeviceToDevice, stream));
-            for (long long si = 0; si < N; ++si) {
-                int e = use_pers ? pers_run(si, si + 1, nullptr)
-                                 : grid_step(wa, wb, si);
-                if (e) return e;
-                if (!use_pers) std::swap(wa, wb);
-                if (nprobe > 0) {
-                    MkAdjT<R> jt = j;
-                    jt.yn = use_pers ? wcur : wa;
-                    int sii = (int)si;
-                    void* params[] = {&a, &jt, &sii};
-                    MKH_HIP(hipModuleLaunchKernel(prog->f_adj_probe, pgrid,
-                                                  1, 1, 64, 1, 1, 0, stream,
-                                                  params, nullptr));
-                }
-            }
-            MKH_HIP(hipStreamSynchronize(stream));
-            if (nprobe > 0)
-                MKH_HIP(hipMemcpy(trace_out, dtrace.p,
-                                  N * nprobe * sizeof(double),
-                                  hipMemcpyDeviceToHost));
-            return 0;
-        }
         if (use_pers)
             MKH_HIP(hipMemcpyAsync(wcur, state[cur].p, SN * sizeof(R),
                                    hipMemcpyDeviceToDevice, stream));
@@ -1624,6 +1632,209 @@ struct Sim : SimBase {
         }
         if (loss) *loss = sl;
         return 0;
+    }
+
+    /* Forward-only run of mkh_adj_trace and mkh_adj_trace_static: V at the
+     * probes after each of the N time steps of tmin..tmax, into trace_out
+     * (host, trace_out[s * nprobe + p]). The run advances a copy of the
+     * state, which it leaves unchanged. The device buffers (step table,
+     * state copies, probe list, trace) are the adjoint's work buffers, or,
+     * when `arena` is given, carved from that block of `arena_bytes` bytes:
+     * then nothing is allocated, and a run that needs more fails with
+     * MKH_E_ARG before anything is launched. */
+    int trace_run(double tmin, double tmax, double dt, const double* events,
+                  long long nev, const long long* probes, long long nprobe,
+                  double* trace_out, long long nrows, char* arena,
+                  size_t arena_bytes)
+    {
+        if (!prog->f_adj_probe)
+            return fail(MKH_E_STATE, "kernel was generated without the "
+                                     "adjoint");
+        if (cfg.coupling == MKH_COUPLING_GRAPH && driver != MKH_DRIVER_GRID &&
+            driver != MKH_DRIVER_BLOCK)
+            return fail(MKH_E_STATE, "graph adjoint needs the grid or block "
+                                     "layout");
+        if (n > INT_MAX)
+            return fail(MKH_E_ARG, "the adjoint supports at most 2^31 - 1 "
+                                   "cells");
+        /* count the steps (the planner is deterministic) */
+        Planner pl;
+        int f = pl.init(tmin, tmax, dt, events, nev, HUGE_VAL);
+        if (f) return pacing_error(f);
+        long long N = 0;
+        {
+            Planner dry = pl;
+            StepRec r;
+            while (!dry.done) {
+                f = dry.next(r);
+                if (f) return pacing_error(f);
+                N++;
+            }
+        }
+        if (N == 0) return fail(MKH_E_ARG, "empty run");
+        if (N > INT_MAX) return fail(MKH_E_ARG, "run too long");
+        if (nrows < N)
+            return fail(MKH_E_ARG, "trace needs one row per time step ("
+                                       + std::to_string(N) + ")");
+        for (long long k = 0; k < nprobe; ++k)
+            if (probes[k] < 0 || probes[k] >= n)
+                return fail(MKH_E_ARG, "probe cell out of range");
+        const size_t SN = (size_t)cfg.n_state * npad;
+        const unsigned bt = (unsigned)cfg.block_threads;
+        const unsigned ggrid = grid_launch_size();
+        const unsigned cgrid = (unsigned)((n + bt - 1) / bt);
+        /* The persistent drivers (sheet, block, cells) advance one scratch
+         * state in place (the sheet: two, it writes its result to the
+         * other); the grid driver steps between two buffers. */
+        const bool use_pers = driver == MKH_DRIVER_SHEET ||
+                              driver == MKH_DRIVER_BLOCK ||
+                              driver == MKH_DRIVER_CELLS;
+        const int nstate = use_pers && driver != MKH_DRIVER_SHEET ? 1 : 2;
+        const size_t ntr = (size_t)std::max<long long>(1, N * nprobe);
+        const size_t npr = (size_t)std::max<long long>(1, nprobe);
+        Step* dsteps = nullptr;
+        R* st[2] = {nullptr, nullptr};
+        long long* dprobes = nullptr;
+        double* dtrace = nullptr;
+        if (arena) {
+            /* regions on 256-byte boundaries, in this order */
+            size_t off = 0;
+            auto carve = [&](size_t bytes) {
+                char* p = arena + off;
+                off += (bytes + 255) / 256 * 256;
+                return p;
+            };
+            dsteps = (Step*)carve((size_t)N * sizeof(Step));
+            for (int k = 0; k < nstate; ++k) st[k] = (R*)carve(SN * sizeof(R));
+            dprobes = (long long*)carve(npr * sizeof(long long));
+            dtrace = (double*)carve(ntr * sizeof(double));
+            if (off > arena_bytes)
+                return fail(MKH_E_ARG,
+                            "trace needs " + std::to_string(off) +
+                                " bytes of scratch; the static scratch holds "
+                                + std::to_string(arena_bytes));
+        } else {
+            MKH_HIP(adj.steps.alloc((size_t)N));
+            MKH_HIP(adj.traj.alloc((size_t)nstate * SN));
+            MKH_HIP(adj.probes.alloc(npr));
+            MKH_HIP(adj.trace.alloc(ntr));
+            dsteps = adj.steps.p;
+            st[0] = adj.traj.p;
+            st[1] = nstate > 1 ? adj.traj.p + SN : nullptr;
+            dprobes = adj.probes.p;
+            dtrace = adj.trace.p;
+        }
+        /* step table, uploaded in chunks from the stack */
+        {
+            Step chunk[512];
+            long long k = 0, base = 0;
+            StepRec r;
+            while (!pl.done) {
+                f = pl.next(r);
+                if (f) return pacing_error(f);
+                Step& s1 = chunk[k++];
+                s1.t = (R)r.t;
+                s1.dt = (R)r.dt;
+                s1.pace = (R)r.pace;
+                s1.row = -1;
+                if (k == 512 || pl.done) {
+                    MKH_HIP(hipMemcpy(dsteps + base, chunk,
+                                      (size_t)k * sizeof(Step),
+                                      hipMemcpyHostToDevice));
+                    base += k;
+                    k = 0;
+                }
+            }
+        }
+        if (nprobe > 0)
+            MKH_HIP(hipMemcpy(dprobes, probes,
+                              (size_t)nprobe * sizeof(long long),
+                              hipMemcpyHostToDevice));
+        const int m1 = -1;
+        MKH_HIP(hipMemcpy(halt.p, &m1, sizeof(int), hipMemcpyHostToDevice));
+        MkAdjT<R> j;
+        std::memset(&j, 0, sizeof(j));
+        j.probes = dprobes;
+        j.n_probe = (int)nprobe;
+        j.target_row = -1;
+        j.trace = dtrace;
+        Args a = base_args();
+        a.steps = dsteps;
+        R* wa = st[0];
+        R* wb = st[1];
+        MKH_HIP(hipMemcpyAsync(wa, state[cur].p, SN * sizeof(R),
+                               hipMemcpyDeviceToDevice, stream));
+        const unsigned pgrid = (unsigned)((nprobe + 63) / 64);
+        for (long long si = 0; si < N; ++si) {
+            int sii = (int)si;
+            if (driver == MKH_DRIVER_SHEET) {
+                Args as = a;
+                as.state_a = wa;
+                as.state_b = wb;
+                MkSheetT<R> sh = sheet_args();
+                sh.g0 = (int)sh_steps;
+                int i0 = sii, i1 = sii + 1;
+                void* params[] = {&as, &sh, &i0, &i1};
+                MKH_HIP(hipModuleLaunchCooperativeKernel(
+                    prog->f_sheet, (unsigned)(sh_tx * sh_ty), 1, 1, 256, 1,
+                    1, 0, stream, params));
+                sh_steps += 1;
+                std::swap(wa, wb);
+            } else if (use_pers) {
+                Args as = a;
+                as.state_a = wa;
+                int i0 = sii, i1 = sii + 1;
+                R* trajp = nullptr;
+                void* params[] = {&as, &i0, &i1, &trajp};
+                const bool blk = driver == MKH_DRIVER_BLOCK;
+                MKH_HIP(hipModuleLaunchKernel(
+                    blk ? prog->f_block : prog->f_cells,
+                    blk ? (unsigned)ndom : cgrid, 1, 1, bt, 1, 1, 0, stream,
+                    params, nullptr));
+            } else {
+                void* params[] = {&a, &wa, &wb, &sii};
+                MKH_HIP(hipModuleLaunchKernel(prog->f_grid, ggrid, 1, 1, bt,
+                                              1, 1, 0, stream, params,
+                                              nullptr));
+                std::swap(wa, wb);
+            }
+            if (nprobe > 0) {
+                MkAdjT<R> jt = j;
+                jt.yn = wa;
+                void* params[] = {&a, &jt, &sii};
+                MKH_HIP(hipModuleLaunchKernel(prog->f_adj_probe, pgrid, 1, 1,
+                                              64, 1, 1, 0, stream, params,
+                                              nullptr));
+            }
+        }
+        MKH_HIP(hipStreamSynchronize(stream));
+        if (driver == MKH_DRIVER_SHEET) {
+            int e = 0;
+            MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
+                              hipMemcpyDeviceToHost));
+            if (e)
+                return fail(MKH_E_HIP, "sheet driver: a neighbour exchange "
+                                       "timed out");
+        }
+        if (nprobe > 0)
+            MKH_HIP(hipMemcpy(trace_out, dtrace,
+                              (size_t)N * nprobe * sizeof(double),
+                              hipMemcpyDeviceToHost));
+        return 0;
+    }
+
+    /* mkh_adj_trace_static: trace_run on this device's static scratch */
+    int trace_static(double tmin, double tmax, double dt,
+                     const double* events, long long nev,
+                     const long long* probes, long long nprobe,
+                     double* trace_out, long long nrows) override
+    {
+        void* base = nullptr;
+        int e = scratch_address(cfg.device, &base);
+        if (e) return e;
+        return trace_run(tmin, tmax, dt, events, nev, probes, nprobe,
+                         trace_out, nrows, (char*)base,
+                         (size_t)MKH_STATIC_SCRATCH_BYTES);
     }
 
     int adj_timing(double* ms) override
@@ -2211,6 +2422,16 @@ int mkh_adj_trace(void* sim, double tmin, double tmax, double dt,
     if (!sim || !trace) return fail(MKH_E_ARG, "null argument");
     return SIM->adj_run(tmin, tmax, dt, events, n_events, probes, n_probes,
                         nullptr, n_rows, 1, nullptr, nullptr, nullptr, trace);
+}
+
+int mkh_adj_trace_static(void* sim, double tmin, double tmax, double dt,
+                         const double* events, long long n_events,
+                         const long long* probes, long long n_probes,
+                         double* trace, long long n_rows)
+{
+    if (!sim || !trace) return fail(MKH_E_ARG, "null argument");
+    return SIM->trace_static(tmin, tmax, dt, events, n_events, probes,
+                             n_probes, trace, n_rows);
 }
 
 int mkh_sim_driver(void* sim, int* driver)
