@@ -58,6 +58,9 @@ extern "C" {
 #define MKH_DRIVER_BLOCK 1  /* one workgroup per domain, all steps per launch */
 #define MKH_DRIVER_GRID 2   /* whole grid, one fused step per launch */
 #define MKH_DRIVER_SHEET 3  /* 2D sheet, persistent tiles, all steps per launch */
+#define MKH_DRIVER_CHAIN 4  /* graphs of disjoint chains (cables) laid end to
+                               end: persistent one-wave tiles, all steps per
+                               launch */
 
 /* Simulation geometry and kernel layout. */
 typedef struct mkh_sim_config {
@@ -73,6 +76,8 @@ typedef struct mkh_sim_config {
     int device;            /* HIP device ordinal */
     int sheet_cpt;         /* cells per thread of the sheet driver (MK_SCPT in
                               the kernel source), 0 = no sheet driver */
+    int chain_cpt;         /* cells per thread of the chain driver (MK_CCPT in
+                              the kernel source), 0 = no chain driver */
 } mkh_sim_config;
 
 /* Library */

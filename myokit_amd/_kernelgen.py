@@ -720,13 +720,29 @@ def sheet_enabled(precision):
     return True
 
 
+def chain_cpt(precision):
+    """Cells per thread of the persistent chain driver: one-wave tiles of up
+    to 64 * cpt cells."""
+    return 2
+
+
+def chain_waves(precision):
+    """Waves per SIMD the chain kernel's register budget must allow: 5 in
+    single precision (so that 5 one-wave tiles of 64 cells per SIMD, 327,680
+    cells, are co-resident on 256 CUs), 3 in double."""
+    return 3 if precision == DOUBLE_PRECISION else 5
+
+
+CHAIN_CPT_CHOICES = (1, 2, 4)
+
+
 SHEET_CPT_CHOICES = (1, 2, 4, 8)
 
 
 def generate_kernel(cm, precision, coupling, block_threads=256,
                     native_maths=False, adjoint=False, tile_cpt=None,
                     tile_waves=None, adjoint_precision=None,
-                    adjoint_waves=None):
+                    adjoint_waves=None, chain_cells=None, chain_budget=None):
     """Returns the complete HIP source for hiprtc. ``tile_cpt`` (cells
     per thread of the persistent sheet: tile height 4 x cpt) and
     ``tile_waves`` (waves per SIMD its register budget allows) default to
@@ -734,7 +750,10 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
     adjoint kernels are appended; ``adjoint_precision`` is the precision of
     the cell VJP's arithmetic (default: ``precision``) and
     ``adjoint_waves`` the waves per SIMD the reverse kernels' register
-    budget allows (0: no limit; default: :func:`adjoint_waves`)."""
+    budget allows (0: no limit; default: :func:`adjoint_waves`).
+    ``chain_cells`` and ``chain_budget`` set the chain driver's cells per
+    thread and waves per SIMD (defaults :func:`chain_cpt`,
+    :func:`chain_waves`)."""
     if block_threads % 64 or not 64 <= block_threads <= 1024:
         raise ValueError('block_threads must be a multiple of 64 in'
                          ' [64, 1024].')
@@ -746,6 +765,14 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
                          + str(SHEET_CPT_CHOICES) + '.')
     if not 1 <= waves <= 8:
         raise ValueError('sheet_waves must be in [1, 8].')
+    ccpt = chain_cpt(precision) if chain_cells is None else int(chain_cells)
+    cwaves = chain_waves(precision) if chain_budget is None \
+        else int(chain_budget)
+    if ccpt not in CHAIN_CPT_CHOICES:
+        raise ValueError('chain_cpt must be one of '
+                         + str(CHAIN_CPT_CHOICES) + '.')
+    if not 1 <= cwaves <= 8:
+        raise ValueError('chain_waves must be in [1, 8].')
     vm = cm.model.label('membrane_potential')
     i_vm = vm.index() if (vm is not None and vm.is_state()) else 0
     bcells = 2048 if precision == DOUBLE_PRECISION else 4096
@@ -763,6 +790,8 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_BCELLS ' + str(bcells),
         '#define MK_SCPT ' + str(cpt),
         '#define MK_SWAVES ' + str(waves),
+        '#define MK_CCPT ' + str(ccpt),
+        '#define MK_CWAVES ' + str(cwaves),
         '#define MK_ADJ_SHEET ' + ('1' if adjoint and int(coupling) == 1
                                    and asg > 0
                                    and adjoint_precision in (None, precision)

@@ -347,6 +347,14 @@ class Model:
     def has_component(self, name):
         return name in self._components
 
+    def __getitem__(self, name):
+        """The component with the given name (``Model.__getitem__``,
+        ``myokit/_model_api.py``)."""
+        return self._components[name]
+
+    def __contains__(self, name):
+        return name in self._components
+
     def get(self, name, class_=None):
         """Returns the component or variable with the given qname."""
         if isinstance(name, (Variable, Component)):

@@ -428,10 +428,10 @@ mk_adj_gather(const double* __restrict__ src,
  * LDS, and the registers are budgeted for as many waves per SIMD. mu of the tile stays in LDS for the whole
  * segment, abar of the tile and a one-cell ring around it too. The
  * potentials V_s and V_{s+1} come from the trajectory, complete in global
- * memory, so only abar crosses tiles: after each step a tile writes the
- * abar of its four edges with agent-scope atomic stores and publishes its
- * step count; before the next step it waits for its neighbours' counts and
- * copies their edges into its ring (the exchange of mk_sheet).
+ * memory, so only abar crosses tiles: after each step a tile stores the
+ * abar of its four edges as step-tagged words (the exchange of mk_sheet),
+ * and before the next step each thread of its ring waits for its
+ * neighbour's value of that step.
  *
  * A step has two phases. Phase 1 computes, for every cell of the tile, the
  * coupling transpose gx (abar_c - abar_x) + gy (...) from abar_{s+1} and
@@ -472,65 +472,73 @@ __device__ __forceinline__ Real mk_adj_sheet_couple(
     return a.gx * ax + a.gy * ay;
 }
 
-__device__ __forceinline__ Real* mk_adj_edge(const MkSheet& sh, int par,
-                                             int tile, int e)
+/* Value k of abar edge e (0 bottom row, 1 top row, 2 left column, 3 right
+ * column) of a tile, slot of step parity par (the layout of mk_edge). */
+__device__ __forceinline__ MkXWord* mk_adj_edge(const MkSheet& sh, int par,
+                                                int tile, int e, int k)
 {
-    return sh.edges + ((long long)(par * sh.tiles_x * sh.tiles_y + tile) * 4
-                       + e) * 64;
+    return sh.edges + (((long long)(par * sh.tiles_x * sh.tiles_y + tile) * 4
+                        + e) * 64 + k) * MK_XW;
 }
 
-/* Waits until the neighbours in nb[4] have published `target` steps.
- * Returns false on a timeout. */
+/* Waits until each neighbour in nb[4] has stored the abar edges of local
+ * step `step` (tagged step; its edge facing this tile, value 0). Returns
+ * false on a timeout. */
 __device__ __forceinline__ bool mk_adj_sheet_wait(const MkSheet& sh,
-                                                  const int nb[4], int target,
+                                                  const int nb[4], int step,
                                                   int* flag)
 {
     if (threadIdx.x < 4) {
+        /* the edge of neighbour t facing this tile is t ^ 1 */
         const int nbt = nb[threadIdx.x];
-        if (nbt >= 0) {
-            int spins = 0;
-            while (__hip_atomic_load(sh.cnt + nbt, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > sh.spin_limit) {
-                    __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    *flag = 1;
-                    break;
-                }
-            }
+        Real v;
+        if (nbt >= 0 &&
+            !mk_tget(mk_adj_edge(sh, (step - 1) & 1, nbt, threadIdx.x ^ 1, 0),
+                     step, sh.spin_limit, v)) {
+            __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            *flag = 1;
         }
     }
     __syncthreads();
     return !*flag;
 }
 
-/* Waits until the neighbours in nb[4] have published `target` steps and
- * copies their abar edges of parity `par` into the ring of at. Returns
- * false on a timeout. */
+/* The neighbours' abar edges of local step `step` (tagged step) into the
+ * ring of at, each ring thread waiting for its own value. Returns false on
+ * a timeout. */
 __device__ __forceinline__ bool mk_adj_sheet_halo(
-    const MkSheet& sh, const int nb[4], int target, int par, Real (*at)[66],
-    int* flag)
+    const MkSheet& sh, const int nb[4], int step, Real (*at)[66], int* flag,
+    int gx0, int gy0, int nx, int ny)
 {
-    if (!mk_adj_sheet_wait(sh, nb, target, flag)) return false;
+    const int par = (step - 1) & 1;
     const int t = threadIdx.x;
+    int nbt = -1, e = 0, k = 0, lr = 0, lc = 0;
     if (t < 64) {
-        if (nb[0] >= 0) at[0][t + 1] = mk_xload(mk_adj_edge(sh, par, nb[0], 1) + t);
+        nbt = nb[0]; e = 1; k = t; lr = 0; lc = t + 1;
     } else if (t < 128) {
-        if (nb[1] >= 0)
-            at[MK_ATH + 1][t - 63] = mk_xload(mk_adj_edge(sh, par, nb[1], 0) + t - 64);
+        nbt = nb[1]; e = 0; k = t - 64; lr = MK_ATH + 1; lc = t - 63;
     } else if (t < 128 + MK_ATH) {
-        if (nb[2] >= 0)
-            at[t - 127][0] = mk_xload(mk_adj_edge(sh, par, nb[2], 3) + t - 128);
+        nbt = nb[2]; e = 3; k = t - 128; lr = t - 127; lc = 0;
     } else if (t < 128 + 2 * MK_ATH) {
-        if (nb[3] >= 0)
-            at[t - 127 - MK_ATH][65] =
-                mk_xload(mk_adj_edge(sh, par, nb[3], 2) + t - 128 - MK_ATH);
+        nbt = nb[3]; e = 2; k = t - 128 - MK_ATH; lr = t - 127 - MK_ATH;
+        lc = 65;
+    }
+    /* only cells of the sheet are written (not the columns of a partial
+     * last tile column or the rows of a partial last tile row) */
+    if (nbt >= 0 && (e < 2 ? gx0 + k < nx : gy0 + k < ny)) {
+        Real v;
+        if (mk_tget(mk_adj_edge(sh, par, nbt, e, k), step, sh.spin_limit, v)) {
+            at[lr][lc] = v;
+        } else {
+            __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            *flag = 1;
+        }
     }
     __syncthreads();
-    return true;
+    return !*flag;
 }
-
 extern "C" __global__ void __launch_bounds__(256, MK_ADJ_SWAVES)
 mk_adj_sheet(const MkArgs a, const MkAdj j, const MkSheet sh, const int s0,
              const int s1, const int flags)
@@ -586,7 +594,8 @@ mk_adj_sheet(const MkArgs a, const MkAdj j, const MkSheet sh, const int s0,
     const int nsteps = s1 - s0;
     for (int i = 0; i < nsteps; ++i) {
         const int s = s1 - 1 - i;
-        if (i > 0 && !mk_adj_sheet_halo(sh, nb, i, (i - 1) & 1, at, &wg_flag))
+        if (i > 0 && !mk_adj_sheet_halo(sh, nb, i, at, &wg_flag, tx * 64, gy0,
+                                        nx, ny))
             return;
         const Real* __restrict__ ys = j.yn + (long long)(s - s0) * SN;
         const Real* __restrict__ V0 = ys + MK_IVM * np;       /* V_s */
@@ -629,24 +638,22 @@ mk_adj_sheet(const MkArgs a, const MkAdj j, const MkSheet sh, const int s0,
 #pragma unroll
             for (int q = 0; q < MK_NS; ++q) mul[q][r][lx] = mu[q];
             at[r + 1][lx + 1] = abo;
-            if (r == 0) mk_xstore(mk_adj_edge(sh, par, tile, 0) + lx, abo);
-            if (r == MK_ATH - 1) mk_xstore(mk_adj_edge(sh, par, tile, 1) + lx, abo);
-            if (lx == 0) mk_xstore(mk_adj_edge(sh, par, tile, 2) + r, abo);
-            if (lx == 63) mk_xstore(mk_adj_edge(sh, par, tile, 3) + r, abo);
+            if (r == 0) mk_tput(mk_adj_edge(sh, par, tile, 0, lx), abo, i + 1);
+            if (r == MK_ATH - 1)
+                mk_tput(mk_adj_edge(sh, par, tile, 1, lx), abo, i + 1);
+            if (lx == 0) mk_tput(mk_adj_edge(sh, par, tile, 2, r), abo, i + 1);
+            if (lx == 63) mk_tput(mk_adj_edge(sh, par, tile, 3, r), abo, i + 1);
 #pragma unroll
             for (int f = 0; f < MK_NF; ++f) j.gfield[f * np + c] += (double)fb[f];
         }
-        /* Edge stores complete, then the step count */
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (threadIdx.x == 0)
-            __hip_atomic_store(sh.cnt + tile, i + 1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        /* the next step's barrier (after its halo) orders this step's
+         * LDS writes before its reads */
     }
     /* mu and abar of step s0 for the previous segment; at the start of the
      * run, the conductance terms of step 0 instead */
     if (first_seg) {
-        if (!mk_adj_sheet_halo(sh, nb, nsteps, (nsteps - 1) & 1, at, &wg_flag))
+        if (!mk_adj_sheet_halo(sh, nb, nsteps, at, &wg_flag, tx * 64, gy0, nx,
+                               ny))
             return;
         const Real* __restrict__ V0 = j.yn + MK_IVM * np;
         for (int k = 0; k < MK_ACPT; ++k) {

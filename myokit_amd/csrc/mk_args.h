@@ -78,11 +78,14 @@ struct MkAdjT {
  * of 64 x (4 * MK_ACPT) cells per 256-thread workgroup. */
 #define MK_ACPT 8
 
-/* Persistent sheet driver: neighbour exchange between workgroup tiles. */
+/* Persistent sheet driver: neighbour exchange between workgroup tiles.
+ * An exchanged value is one 64-bit word per 32 bits of the value, each word
+ * holding the step it belongs to (high half) and 32 bits of the value (low
+ * half). */
 template <typename R>
 struct MkSheetT {
-    R* edges;         /* [2][tiles][4][64] tile edge potentials (by parity) */
-    int* cnt;         /* [tiles] steps completed by each tile */
+    unsigned long long* edges;  /* [2][tiles][4][64][sizeof(R) / 4] tile
+                                   edge values, tagged (by step parity) */
     int* err;         /* set to 1 if a neighbour wait timed out */
     int tiles_x;      /* tiles along x (64 cells each) */
     int tiles_y;      /* tiles along y (MK_STH cells each) */
@@ -90,6 +93,26 @@ struct MkSheetT {
     int spin_limit;   /* polls before a wait is abandoned */
     R* traj;          /* if set: the state after local step i is stored at
                          traj + i * n_state * npad (SoA), for the adjoint */
+};
+
+/* Persistent chain driver: cells of disjoint chains (cables) laid end to
+ * end, tile_len (at most 64 * MK_CCPT) consecutive cells per one-wave
+ * workgroup. */
+template <typename R>
+struct MkChainT {
+    const R* gl;              /* [npad] conductance to cell c - 1 */
+    const R* gr;              /* [npad] conductance to cell c + 1 */
+    const unsigned char* cf;  /* [npad] bit 0: connected to c - 1, bit 1:
+                                 to c + 1, bit 2: the c - 1 connection comes
+                                 first in the cell's connection-list order */
+    unsigned long long* edges;  /* [2][tiles][2][sizeof(R) / 4] first and
+                                   last V of each tile, tagged as the
+                                   sheet's (by step parity) */
+    int* err;                 /* set to 1 if a neighbour wait timed out */
+    int tiles;                /* number of tiles */
+    int tile_len;             /* cells per tile (the last may have fewer) */
+    int g0;                   /* steps completed before this launch */
+    int spin_limit;           /* polls before a wait is abandoned */
 };
 
 #endif

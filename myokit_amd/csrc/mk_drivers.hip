@@ -17,7 +17,7 @@
  * with the state at t, the intermediaries and idiff written to the log at
  * log steps. The reference runs this as 2-3 kernels per step with the state
  * in AoS layout; here the coupling and the update are fused in one pass over
- * SoA state, and four drivers cover the geometries:
+ * SoA state, and five drivers cover the geometries:
  *
  *   mk_cells  uncoupled cells: one thread per cell keeps its state in
  *             registers across every step of the launch (no synchronisation);
@@ -30,7 +30,10 @@
  *   mk_sheet  large homogeneous 2D sheets: a persistent cooperative launch,
  *             one 64 x 16 tile per workgroup with its state in registers
  *             for every step of the launch; tile edges are exchanged
- *             through L2 with a step counter per tile.
+ *             through L2 with a step counter per tile;
+ *   mk_chain  graphs of disjoint chains (cables): the same persistent
+ *             scheme over the cells laid end to end, a cable spread over
+ *             as many tiles as its length needs.
  *
  * Halting: at a log step, the thread of the caller's cell 0 (nan_cell in
  * the device's cell order) records the log row in halt[0] if state 0 is NaN
@@ -351,12 +354,11 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
  * the full state of cells (x0 + lx, y0 + w * MK_SCPT + k), k < MK_SCPT, in
  * registers. The tile's membrane potentials, plus a one-cell halo ring, live
  * in LDS (double-buffered by step). After each step a tile writes its four
- * edge rows/columns of V to global memory with agent-scope atomic stores,
- * waits for their completion and publishes its step count (agent-scope
- * atomic store); before the next step it polls the step counts of its four
- * neighbours and copies their edges (agent-scope atomic loads) into its halo
- * ring. No state goes to HBM inside a launch: per cell-step the only global
- * traffic is the edge exchange.
+ * edge rows/columns of V to global memory as step-tagged words; before the
+ * next step each thread of its halo ring waits for its neighbour value of
+ * that step and copies it into LDS (one barrier per step). No state goes to
+ * HBM inside a launch: per cell-step the only global traffic is the edge
+ * exchange.
  *
  * The launch is cooperative, so all tiles are resident at once. Every wait
  * is bounded (spin_limit polls); on a timeout the tile records an error and
@@ -371,26 +373,71 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
 #define MK_STH (4 * MK_SCPT)
 typedef MkSheetT<Real> MkSheet;
 
-/* Exchanged values are written and read with agent-scope atomic accesses:
- * these are coherent across the XCDs' separate L2 caches (write-through on
- * store, L2 miss on load), so the exchange needs no L2-wide write-back or
- * invalidation, which as a per-step fence cost more than the step itself. */
-__device__ __forceinline__ void mk_xstore(Real* p, Real v)
+/* Exchange between tiles. A value crosses as one 64-bit word per 32 bits
+ * (one in single precision, two in double), each word holding the step the
+ * value belongs to (high half) and 32 bits of the value (low half), written
+ * and read with agent-scope atomic accesses of the whole word. A reader
+ * polls the word until its step is the one it needs: an atomic access of
+ * one location sees one whole write of it, so no ordering between
+ * different locations (a separate flag, a fence) is needed, and a writer
+ * never waits for its stores. Atomics at agent scope are coherent across
+ * the XCDs' separate L2 caches. Each value has two slots, by step parity:
+ * a writer reuses a slot two steps later, which needs its neighbour's value
+ * of the step in between, which the neighbour writes only after it has
+ * read the slot. */
+typedef unsigned long long MkXWord;
+#define MK_XW ((int)(sizeof(Real) / 4))
+
+__device__ __forceinline__ void mk_tput(MkXWord* p, Real v, int step)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const MkXWord t = (MkXWord)(unsigned)step << 32;
+    if constexpr (sizeof(Real) == 4) {
+        __hip_atomic_store(p, t | (MkXWord)__float_as_uint((float)v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        const MkXWord b = (MkXWord)__double_as_longlong((double)v);
+        __hip_atomic_store(p, t | (b >> 32), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p + 1, t | (b & 0xffffffffull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
-__device__ __forceinline__ Real mk_xload(const Real* p)
+/* Polls the value at p until it belongs to `step` (or a later one);
+ * returns false after spin_limit polls. */
+__device__ __forceinline__ bool mk_tget(const MkXWord* p, int step,
+                                        int spin_limit, Real& v)
 {
-    return __hip_atomic_load(const_cast<Real*>(p), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    MkXWord a = 0, b = 0;
+    for (int spins = 0;; ++spins) {
+        a = __hip_atomic_load(const_cast<MkXWord*>(p), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+        bool ready = (int)(a >> 32) >= step;
+        if constexpr (sizeof(Real) == 8) {
+            b = __hip_atomic_load(const_cast<MkXWord*>(p + 1),
+                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ready = ready && (int)(b >> 32) >= step;
+        }
+        if (ready) break;
+        if (spins >= spin_limit) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if constexpr (sizeof(Real) == 4) {
+        v = (Real)__uint_as_float((unsigned)a);
+    } else {
+        v = (Real)__longlong_as_double(
+            (long long)(((a & 0xffffffffull) << 32) | (b & 0xffffffffull)));
+    }
+    return true;
 }
 
-__device__ __forceinline__ Real* mk_edge(const MkSheet& sh, int par, int tile,
-                                         int e)
+/* Value k of edge e (0 bottom row, 1 top row, 2 left column, 3 right
+ * column) of a tile, slot of step parity par. */
+__device__ __forceinline__ MkXWord* mk_edge(const MkSheet& sh, int par,
+                                            int tile, int e, int k)
 {
-    return sh.edges + ((long long)(par * sh.tiles_x * sh.tiles_y + tile) * 4
-                       + e) * 64;
+    return sh.edges + (((long long)(par * sh.tiles_x * sh.tiles_y + tile) * 4
+                        + e) * 64 + k) * MK_XW;
 }
 
 extern "C" __global__ void __launch_bounds__(256, MK_SWAVES)
@@ -462,49 +509,41 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
     for (int s = s0; s < s1; ++s) {
         const int i = s - s0;
         if (i > 0) {
-            /* Wait for the neighbours' previous step, then read their edges
-             * (south: its top row, north: its bottom row, west: its right
-             * column, east: its left column) into the halo ring. */
-            const int target = sh.g0 + i;
-            if (threadIdx.x < 4) {
-                const int nbt = nb_tile[threadIdx.x];
-                if (nbt >= 0) {
-                    int spins = 0;
-                    while (__hip_atomic_load(sh.cnt + nbt, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT)
-                           < target) {
-                        __builtin_amdgcn_s_sleep(1);
-                        if (++spins > sh.spin_limit) {
-                            __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                            wg_flag = 1;
-                            break;
-                        }
-                    }
+            /* The neighbours' edges of the previous step (south: its top
+             * row, north: its bottom row, west: its right column, east: its
+             * left column) into the halo ring, each halo thread waiting for
+             * its own value */
+            const int step = sh.g0 + i;
+            const int par = (step - 1) & 1;
+            const int t = threadIdx.x;
+            int nbt = -1, e = 0, k = 0, lr = 0, lc = 0;
+            if (t < 64) {
+                nbt = nb_tile[0]; e = 1; k = t; lr = 0; lc = t + 1;
+            } else if (t < 128) {
+                nbt = nb_tile[1]; e = 0; k = t - 64; lr = MK_STH + 1;
+                lc = t - 63;
+            } else if (t < 128 + MK_STH) {
+                nbt = nb_tile[2]; e = 3; k = t - 128; lr = t - 127; lc = 0;
+            } else if (t < 128 + 2 * MK_STH) {
+                nbt = nb_tile[3]; e = 2; k = t - 128 - MK_STH;
+                lr = t - 127 - MK_STH; lc = 65;
+            }
+            /* only cells of the sheet are written: the columns of a partial
+             * last tile column, the rows of a partial last tile row, are
+             * not */
+            if (nbt >= 0 && (e < 2 ? gx0 + k < nx : gy0 + k < ny)) {
+                Real v;
+                if (mk_tget(mk_edge(sh, par, nbt, e, k), step, sh.spin_limit,
+                            v)) {
+                    vt[cur][lr][lc] = v;
+                } else {
+                    __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    wg_flag = 1;
                 }
             }
             __syncthreads();
             if (wg_flag) return;
-            const int par = (sh.g0 + i - 1) & 1;
-            const int t = threadIdx.x;
-            if (t < 64) {
-                if (nb_tile[0] >= 0)
-                    vt[cur][0][t + 1] =
-                        mk_xload(mk_edge(sh, par, nb_tile[0], 1) + t);
-            } else if (t < 128) {
-                if (nb_tile[1] >= 0)
-                    vt[cur][MK_STH + 1][t - 63] =
-                        mk_xload(mk_edge(sh, par, nb_tile[1], 0) + t - 64);
-            } else if (t < 128 + MK_STH) {
-                if (nb_tile[2] >= 0)
-                    vt[cur][t - 127][0] =
-                        mk_xload(mk_edge(sh, par, nb_tile[2], 3) + t - 128);
-            } else if (t < 128 + 2 * MK_STH) {
-                if (nb_tile[3] >= 0)
-                    vt[cur][t - 127 - MK_STH][65] = mk_xload(
-                        mk_edge(sh, par, nb_tile[3], 2) + t - 128 - MK_STH);
-            }
-            __syncthreads();
         }
         const MkStep p = a.steps[s];
         const bool last = s + 1 == s1;
@@ -543,24 +582,18 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
                 const Real vn = y[k][MK_IVM];
                 vt[cur ^ 1][r + 1][lx + 1] = vn;
                 if (!last) {
-                    if (r == 0) mk_xstore(mk_edge(sh, par, tile, 0) + lx, vn);
+                    const int step = sh.g0 + i + 1;
+                    if (r == 0) mk_tput(mk_edge(sh, par, tile, 0, lx), vn, step);
                     if (r == MK_STH - 1)
-                        mk_xstore(mk_edge(sh, par, tile, 1) + lx, vn);
-                    if (lx == 0) mk_xstore(mk_edge(sh, par, tile, 2) + r, vn);
-                    if (lx == 63) mk_xstore(mk_edge(sh, par, tile, 3) + r, vn);
+                        mk_tput(mk_edge(sh, par, tile, 1, lx), vn, step);
+                    if (lx == 0) mk_tput(mk_edge(sh, par, tile, 2, r), vn, step);
+                    if (lx == 63)
+                        mk_tput(mk_edge(sh, par, tile, 3, r), vn, step);
                 }
             }
         }
-        if (!last) {
-            /* Every wave waits until its edge stores are complete (written
-             * through to memory at agent scope), then one thread publishes
-             * the step count. */
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            if (threadIdx.x == 0)
-                __hip_atomic_store(sh.cnt + tile, sh.g0 + i + 1,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        /* The next step's barrier (after its halo) orders this step's LDS
+         * writes before its reads */
         cur ^= 1;
     }
     /* The final state goes to the other buffer: a neighbour tile may still
@@ -575,10 +608,178 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
                 a.state_b[(long long)i * a.npad + c] = y[k][i];
         }
     }
-    /* The final step count is published with the state at kernel end */
+}
+
+/* ------------------------------------------------------------------------
+ * Persistent chains: disjoint chains of cells (cables built with
+ * set_connections) laid end to end in the device's cell order, every
+ * connection joining cells c and c + 1 (diff_arb_step, openclsim.cl:537-556,
+ * for graphs of that shape).
+ *
+ * Each workgroup is one wave (64 threads) and owns ch.tile_len consecutive
+ * cells (at most 64 * MK_CCPT; the last tile may be shorter) for every step
+ * of the launch, thread t the cells t + 64 k, with their full state in
+ * registers. One-wave tiles balance the work over the CUs to within one
+ * wave. The tile's membrane potentials and one halo cell on each side live
+ * in LDS (double-buffered by step). A cable longer than a tile spans
+ * several: after each step a tile stores its first and last V as
+ * step-tagged words (the exchange of mk_sheet); before the next step it
+ * waits (bounded) for the value of each neighbour it is connected to. Tiles
+ * whose boundary falls between two cables neither wait nor read there. Where the block driver gives a
+ * whole cable to one workgroup, so that the longest cable sets the
+ * launch's duration, here every cable is spread over as many tiles as its
+ * length needs.
+ *
+ * A cell's diffusion current sums its (at most two) connection terms in the
+ * cell's connection-list order, as mk_graph does.
+ * ---------------------------------------------------------------------- */
+#ifndef MK_CCPT
+#define MK_CCPT 2
+#endif
+#ifndef MK_CWAVES
+#define MK_CWAVES 4     /* waves per SIMD the register budget must allow */
+#endif
+#define MK_CT (64 * MK_CCPT)
+typedef MkChainT<Real> MkChain;
+
+/* Edge e (0 first cell, 1 last cell) of a tile, slot of step parity par. */
+__device__ __forceinline__ MkXWord* mk_cedge(const MkChain& ch, int par,
+                                             int tile, int e)
+{
+    return ch.edges + (((long long)par * ch.tiles + tile) * 2 + e) * MK_XW;
+}
+
+extern "C" __global__ void __launch_bounds__(64, MK_CWAVES)
+mk_chain(const MkArgs a, const MkChain ch, const int s0, const int s1)
+{
+    __shared__ Real vc[2][MK_CT + 2];
+    __shared__ int wg_flag, need[2];
     if (threadIdx.x == 0)
-        __hip_atomic_store(sh.cnt + tile, sh.g0 + (s1 - s0), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        wg_flag = mk_halted(a, a.steps[s1 - 1].row) ? 1 : 0;
+    __syncthreads();
+    if (wg_flag) return;
+
+    const int tile = blockIdx.x;
+    const long long c0 = (long long)tile * ch.tile_len;
+    const int len = (int)(a.n - c0 < ch.tile_len ? a.n - c0 : ch.tile_len);
+    const int t = threadIdx.x;
+
+    Real y[MK_CCPT][MK_NS];
+    Real fv[MK_CCPT][MK_NFA];
+    Real gl[MK_CCPT], gr[MK_CCPT];
+    unsigned cf[MK_CCPT];
+    bool act[MK_CCPT], paced[MK_CCPT];
+    Real q[MK_NQA];
+#pragma unroll
+    for (int k = 0; k < MK_CCPT; ++k) {
+        const int li = t + 64 * k;
+        const long long c = c0 + li;
+        act[k] = li < len;
+        const long long cc = act[k] ? c : 0;
+        paced[k] = act[k] && mk_is_paced(a, cc, cc % a.nx, cc / a.nx);
+#pragma unroll
+        for (int i = 0; i < MK_NS; ++i)
+            y[k][i] = act[k] ? a.state_a[(long long)i * a.npad + c] : (Real)0;
+#pragma unroll
+        for (int f = 0; f < MK_NF; ++f)
+            fv[k][f] = act[k] ? a.fields[(long long)f * a.npad + c] : (Real)0;
+        gl[k] = act[k] ? ch.gl[c] : (Real)0;
+        gr[k] = act[k] ? ch.gr[c] : (Real)0;
+        cf[k] = act[k] ? (unsigned)ch.cf[c] : 0u;
+        if (act[k]) vc[0][li + 1] = y[k][MK_IVM];
+        /* the neighbours this tile exchanges with: its first cell connected
+         * to the previous tile's last, its last cell to the next tile's
+         * first */
+        if (li == 0) need[0] = tile > 0 && (cf[k] & 1u);
+        if (li == len - 1) need[1] = tile + 1 < ch.tiles && (cf[k] & 2u);
+    }
+    __syncthreads();
+    const bool nl = need[0] != 0, nr = need[1] != 0;
+    /* halo at the start of the launch, from the global state */
+    {
+        const Real* V = a.state_a + (long long)MK_IVM * a.npad;
+        if (t == 0) vc[0][0] = nl ? V[c0 - 1] : (Real)0;
+        if (t == 1) vc[0][len + 1] = nr ? V[c0 + len] : (Real)0;
+    }
+    __syncthreads();
+
+    int cur = 0;
+    for (int s = s0; s < s1; ++s) {
+        const int i = s - s0;
+        if (i > 0 && (nl || nr)) {
+            /* the connected neighbours' edge cells of the previous step
+             * into the halo */
+            const int step = ch.g0 + i;
+            const int par = (step - 1) & 1;
+            if ((t == 0 && nl) || (t == 1 && nr)) {
+                Real v;
+                if (mk_tget(t == 0 ? mk_cedge(ch, par, tile - 1, 1)
+                                   : mk_cedge(ch, par, tile + 1, 0),
+                            step, ch.spin_limit, v)) {
+                    vc[cur][t == 0 ? 0 : len + 1] = v;
+                } else {
+                    __hip_atomic_store(ch.err, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    wg_flag = 1;
+                }
+            }
+            __syncthreads();
+            if (wg_flag) return;
+        }
+        const MkStep p = a.steps[s];
+        const bool last = s + 1 == s1;
+        const int par = (ch.g0 + i) & 1;
+#pragma unroll
+        for (int k = 0; k < MK_CCPT; ++k) {
+            const int li = t + 64 * k;
+            if (act[k]) {
+                const long long c = c0 + li;
+                const Real v = vc[cur][li + 1];
+                const Real vl = vc[cur][li];
+                const Real vr = vc[cur][li + 2];
+                /* the connection terms in list order (mk_graph's sums) */
+                Real id = (Real)0;
+                if (cf[k] & 4u) {
+                    if (cf[k] & 1u) id = id + gl[k] * (v - vl);
+                    if (cf[k] & 2u) id = id + gr[k] * (v - vr);
+                } else {
+                    if (cf[k] & 2u) id = id + gr[k] * (v - vr);
+                    if (cf[k] & 1u) id = id + gl[k] * (v - vl);
+                }
+                const Real pace = paced[k] ? p.pace : (Real)0;
+                if (p.row >= 0 && a.nan_check && c == a.nan_cell &&
+                    y[k][0] != y[k][0])
+                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+                /* One instance of the cell code: logged values are register
+                 * copies, written out only at log steps */
+                mk_cell<true>(y[k], p.t, p.dt, pace, id, fv[k], q);
+                if (p.row >= 0) mk_write_log(a, c, p.row, q);
+                const Real vn = y[k][MK_IVM];
+                vc[cur ^ 1][li + 1] = vn;
+                if (!last) {
+                    if (li == 0 && nl)
+                        mk_tput(mk_cedge(ch, par, tile, 0), vn, ch.g0 + i + 1);
+                    if (li == len - 1 && nr)
+                        mk_tput(mk_cedge(ch, par, tile, 1), vn, ch.g0 + i + 1);
+                }
+            }
+        }
+        /* LDS writes of this step before the next step's reads (with a
+         * neighbour to wait for, the barrier after the halo does it) */
+        if (!(nl || nr)) __syncthreads();
+        cur ^= 1;
+    }
+    /* the final state goes to the other buffer: a neighbour may still be
+     * reading its launch-start halo from state_a. The host swaps them. */
+#pragma unroll
+    for (int k = 0; k < MK_CCPT; ++k) {
+        if (act[k]) {
+            const long long c = c0 + t + 64 * k;
+#pragma unroll
+            for (int i = 0; i < MK_NS; ++i)
+                a.state_b[(long long)i * a.npad + c] = y[k][i];
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------

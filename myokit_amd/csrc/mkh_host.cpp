@@ -70,6 +70,7 @@ struct Program {
     hipModule_t module = nullptr;
     hipFunction_t f_cells = nullptr, f_block = nullptr, f_grid = nullptr;
     hipFunction_t f_sheet = nullptr, f_junction = nullptr;
+    hipFunction_t f_chain = nullptr;
     hipFunction_t f_adj_step = nullptr, f_adj_block = nullptr,
                   f_adj_probe = nullptr, f_adj_sheet = nullptr,
                   f_adj_gather = nullptr;
@@ -201,6 +202,14 @@ int scratch_address(int device, void** ptr)
     return 0;
 }
 
+/* Sets a driver variable for the lifetime of the guard. */
+struct DriverAs {
+    int& d;
+    int saved;
+    DriverAs(int& d_, int as) : d(d_), saved(d_) { d = as; }
+    ~DriverAs() { d = saved; }
+};
+
 /* -------------------------------------------------------------------------
  * Simulation
  * ---------------------------------------------------------------------- */
@@ -289,8 +298,18 @@ struct Sim : SimBase {
     int driver = MKH_DRIVER_GRID;
     long long ndom = 1;
     /* sheet driver */
-    DevBuf<R> sh_edges;
-    DevBuf<int> sh_cnt, sh_err;
+    /* tile edges as step-tagged words: one per 32 bits of a value */
+    static constexpr int XW = (int)(sizeof(R) / 4);
+    DevBuf<unsigned long long> sh_edges;
+    DevBuf<int> sh_err;
+    /* chain driver (graphs whose every connection joins cells c, c + 1) */
+    DevBuf<R> ch_gl, ch_gr;
+    DevBuf<unsigned long long> ch_edges;
+    DevBuf<unsigned char> ch_cf;
+    DevBuf<int> ch_err;
+    bool is_chain = false;
+    int ch_tiles = 0, ch_len = 0;
+    long long ch_steps = 0;
     int sh_tx = 0, sh_ty = 0, sh_capacity = 0;
     long long sh_steps = 0;
     long long max_domain = 0;
@@ -380,10 +399,10 @@ struct Sim : SimBase {
             sh_ty = (int)((cfg.ny + th - 1) / th);
             if ((long long)sh_tx * sh_ty <= sh_capacity) {
                 const size_t nt = (size_t)sh_tx * sh_ty;
-                MKH_HIP(sh_edges.alloc(2 * nt * 4 * 64));
-                MKH_HIP(sh_cnt.alloc(nt));
+                MKH_HIP(sh_edges.alloc(2 * nt * 4 * 64 * XW));
                 MKH_HIP(sh_err.alloc(1));
-                MKH_HIP(hipMemset(sh_cnt.p, 0, nt * sizeof(int)));
+                MKH_HIP(hipMemset(sh_edges.p, 0,
+                                  sh_edges.n * sizeof(unsigned long long)));
                 MKH_HIP(hipMemset(sh_err.p, 0, sizeof(int)));
             }
         }
@@ -404,9 +423,12 @@ struct Sim : SimBase {
             return max_domain <= max_block_cells;
         case MKH_DRIVER_GRID:
             return true;
+        case MKH_DRIVER_CHAIN:
+            return cfg.coupling == MKH_COUPLING_GRAPH && is_chain &&
+                   ch_tiles > 0 && ch_edges.p;
         case MKH_DRIVER_SHEET:
             return cfg.coupling == MKH_COUPLING_STENCIL && cfg.ny > 1 &&
-                   sh_tx > 0 && sh_ty > 0 && sh_cnt.p &&
+                   sh_tx > 0 && sh_ty > 0 && sh_edges.p &&
                    (long long)sh_tx * sh_ty <= sh_capacity;
         default:
             return false;
@@ -551,8 +573,81 @@ struct Sim : SimBase {
         max_domain = 0;
         for (long long c = 0; c < n; ++c)
             max_domain = std::max(max_domain, comp_size[c]);
+        int e = build_chain_layout();
+        if (e) return e;
+        if (forced_driver >= 0 && forced_driver != MKH_DRIVER_GRID &&
+            !feasible(forced_driver))
+            return fail(MKH_E_ARG, "driver " + std::to_string(forced_driver) +
+                                       " cannot run this geometry and "
+                                       "coupling");
         choose_driver_graph();
         return build_graph_layout(root, comp_size);
+    }
+
+    /* The chain layout: used when every connection joins cells c and c + 1
+     * (each at most once), i.e. the graph is a set of disjoint chains laid
+     * end to end in the device's cell order. Per cell: the conductances to
+     * c - 1 and c + 1, which of them exist, and which comes first in the
+     * cell's connection-list order (the order of its sum). The cells are cut
+     * into one-wave tiles of equal length (the last one shorter). */
+    int build_chain_layout()
+    {
+        is_chain = false;
+        ch_tiles = ch_len = 0;
+        const long long m = (long long)ci.size();
+        if (m == 0 || !prog->f_chain || cfg.chain_cpt < 1) return 0;
+        std::vector<R> gl(npad, (R)0), gr(npad, (R)0);
+        std::vector<unsigned char> cf(npad, 0);
+        for (long long k = 0; k < m; ++k) {
+            const long long a = std::min(ci[k], cj[k]);
+            const long long b = std::max(ci[k], cj[k]);
+            if (b != a + 1 || (cf[a] & 2u)) return 0;
+            gr[a] = (R)cg[k];
+            gl[b] = (R)cg[k];
+            /* the first connection listed for a cell sets its order */
+            if (!(cf[a] & 1u)) cf[a] &= (unsigned char)~4u;
+            if (!(cf[b] & 2u)) cf[b] |= 4u;
+            cf[a] |= 2u;
+            cf[b] |= 1u;
+        }
+        int per_cu = 0, ncu = 0;
+        MKH_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, prog->f_chain, 64, 0));
+        MKH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                      cfg.device));
+        const long long cap = (long long)per_cu * ncu;
+        /* 64 * MK_CCPT cells per tile, fewer if the tiles would then not
+         * give every SIMD a wave; more tiles than can be co-resident: as
+         * many equal tiles as can */
+        const long long want = 64LL * cfg.chain_cpt;
+        const long long simds = 4LL * ncu;
+        long long len = std::min<long long>(
+            want, std::max<long long>(64, ((n + simds - 1) / simds + 63) / 64
+                                               * 64));
+        if ((n + len - 1) / len > cap) {
+            if (cap < 1) return 0;
+            len = (n + cap - 1) / cap;
+        }
+        if (len > want) return 0;
+        const long long tiles = (n + len - 1) / len;
+        MKH_HIP(ch_gl.alloc(npad));
+        MKH_HIP(ch_gr.alloc(npad));
+        MKH_HIP(ch_cf.alloc(npad));
+        MKH_HIP(hipMemcpy(ch_gl.p, gl.data(), npad * sizeof(R),
+                          hipMemcpyHostToDevice));
+        MKH_HIP(hipMemcpy(ch_gr.p, gr.data(), npad * sizeof(R),
+                          hipMemcpyHostToDevice));
+        MKH_HIP(hipMemcpy(ch_cf.p, cf.data(), npad, hipMemcpyHostToDevice));
+        MKH_HIP(ch_edges.alloc((size_t)tiles * 4 * XW));
+        MKH_HIP(ch_err.alloc(1));
+        MKH_HIP(hipMemset(ch_edges.p, 0,
+                          ch_edges.n * sizeof(unsigned long long)));
+        MKH_HIP(hipMemset(ch_err.p, 0, sizeof(int)));
+        ch_steps = 0;
+        ch_tiles = (int)tiles;
+        ch_len = (int)len;
+        is_chain = true;
+        return 0;
     }
 
     void choose_driver_graph()
@@ -560,12 +655,17 @@ struct Sim : SimBase {
         ndom = 0;   /* set by build_graph_layout */
         if (forced_driver == MKH_DRIVER_GRID ||
             (forced_driver == MKH_DRIVER_BLOCK &&
-             feasible(MKH_DRIVER_BLOCK))) {
+             feasible(MKH_DRIVER_BLOCK)) ||
+            (forced_driver == MKH_DRIVER_CHAIN &&
+             feasible(MKH_DRIVER_CHAIN))) {
             driver = forced_driver;
             return;
         }
-        driver = max_domain <= max_block_cells ? MKH_DRIVER_BLOCK
-                                               : MKH_DRIVER_GRID;
+        if (feasible(MKH_DRIVER_CHAIN) && forced_driver < 0)
+            driver = MKH_DRIVER_CHAIN;
+        else
+            driver = max_domain <= max_block_cells ? MKH_DRIVER_BLOCK
+                                                   : MKH_DRIVER_GRID;
     }
 
     int build_graph_layout(const std::vector<long long>& root,
@@ -777,10 +877,18 @@ struct Sim : SimBase {
             MKH_HIP(logbuf.alloc((size_t)cap * ncols));
         }
         log_rows_cap_run = cap;
-        if (sh_cnt.p) {
-            MKH_HIP(hipMemset(sh_cnt.p, 0, sh_cnt.n * sizeof(int)));
+        /* the step tags restart: old tags must not pass for new ones */
+        if (sh_edges.p) {
+            MKH_HIP(hipMemset(sh_edges.p, 0,
+                              sh_edges.n * sizeof(unsigned long long)));
             MKH_HIP(hipMemset(sh_err.p, 0, sizeof(int)));
             sh_steps = 0;
+        }
+        if (ch_edges.p) {
+            MKH_HIP(hipMemset(ch_edges.p, 0,
+                              ch_edges.n * sizeof(unsigned long long)));
+            MKH_HIP(hipMemset(ch_err.p, 0, sizeof(int)));
+            ch_steps = 0;
         }
         size_t need = (size_t)batch_steps;
         MKH_HIP(steps.alloc(need));
@@ -844,13 +952,27 @@ struct Sim : SimBase {
         MkSheetT<R> sh;
         std::memset(&sh, 0, sizeof(sh));
         sh.edges = sh_edges.p;
-        sh.cnt = sh_cnt.p;
         sh.err = sh_err.p;
         sh.tiles_x = sh_tx;
         sh.tiles_y = sh_ty;
-        sh.spin_limit = 1 << 22;
+        sh.spin_limit = 1 << 20;
         sh.traj = nullptr;
         return sh;
+    }
+
+    MkChainT<R> chain_args()
+    {
+        MkChainT<R> ch;
+        std::memset(&ch, 0, sizeof(ch));
+        ch.gl = ch_gl.p;
+        ch.gr = ch_gr.p;
+        ch.cf = ch_cf.p;
+        ch.edges = ch_edges.p;
+        ch.err = ch_err.p;
+        ch.tiles = ch_tiles;
+        ch.tile_len = ch_len;
+        ch.spin_limit = 1 << 20;
+        return ch;
     }
 
     hipEvent_t event(size_t k)
@@ -965,14 +1087,17 @@ struct Sim : SimBase {
                                stream));
         MKH_HIP(hipStreamSynchronize(stream));
         MKH_HIP(hipStreamSynchronize(copy_stream));
-        if (driver == MKH_DRIVER_SHEET) {
+        if (driver == MKH_DRIVER_SHEET || driver == MKH_DRIVER_CHAIN) {
             int e = 0;
-            MKH_HIP(hipMemcpy(&e, sh_err.p, sizeof(int),
-                              hipMemcpyDeviceToHost));
+            MKH_HIP(hipMemcpy(&e, driver == MKH_DRIVER_SHEET ? sh_err.p
+                                                             : ch_err.p,
+                              sizeof(int), hipMemcpyDeviceToHost));
             if (e) {
                 running = false;
-                return fail(MKH_E_HIP, "sheet driver: a neighbour exchange "
-                                       "timed out");
+                return fail(MKH_E_HIP, std::string(driver == MKH_DRIVER_SHEET
+                                                       ? "sheet" : "chain") +
+                                           " driver: a neighbour exchange "
+                                           "timed out");
             }
         }
         valid = b.nr;
@@ -1057,6 +1182,33 @@ struct Sim : SimBase {
                     if (e) return e;
                 }
             }
+        } else if (driver == MKH_DRIVER_CHAIN) {
+            MkChainT<R> ch = chain_args();
+            long long s0 = 0;
+            for (long long s = 0; s < ns; ++s) {
+                if (h_steps[s].row >= 0 || s == ns - 1) {
+                    /* reads state[cur], writes state[cur ^ 1] */
+                    Args a = base_args();
+                    int i0 = (int)s0, i1 = (int)(s + 1);
+                    ch.g0 = (int)ch_steps;
+                    void* params[] = {&a, &ch, &i0, &i1};
+                    if (time_kernels)
+                        MKH_HIP(hipEventRecord(event(b.nev), stream));
+                    MKH_HIP(hipModuleLaunchCooperativeKernel(
+                        prog->f_chain, (unsigned)ch_tiles, 1, 1, 64, 1, 1, 0,
+                        stream, params));
+                    if (time_kernels) {
+                        MKH_HIP(hipEventRecord(event(b.nev + 1), stream));
+                        b.nev += 2;
+                    }
+                    launches++;
+                    ch_steps += i1 - i0;
+                    cur ^= 1;
+                    s0 = s + 1;
+                    e = emit_row(b, s);
+                    if (e) return e;
+                }
+            }
         } else {
             hipFunction_t f = driver == MKH_DRIVER_CELLS ? prog->f_cells
                                                          : prog->f_block;
@@ -1101,10 +1253,17 @@ struct Sim : SimBase {
         } else if (k == "time_kernels") {
             time_kernels = v != 0;
         } else if (k == "driver") {
-            if (v < -1 || v > MKH_DRIVER_SHEET ||
-                (v >= 0 && !feasible((int)v)) ||
+            if (cfg.coupling == MKH_COUPLING_GRAPH && !have_graph &&
+                (v == MKH_DRIVER_BLOCK || v == MKH_DRIVER_CHAIN)) {
+                /* checked against the graph when it arrives */
+                forced_driver = (int)v;
+                return 0;
+            }
+            if (v < -1 || v > MKH_DRIVER_CHAIN ||
+                (v >= 0 && v != MKH_DRIVER_CHAIN && !feasible((int)v)) ||
+                (v == MKH_DRIVER_CHAIN && !is_chain) ||
                 (have_graph && v >= 0 && v != MKH_DRIVER_GRID &&
-                 v != MKH_DRIVER_BLOCK))
+                 v != MKH_DRIVER_BLOCK && v != MKH_DRIVER_CHAIN))
                 return fail(MKH_E_ARG, "driver " + std::to_string(v) +
                                            " cannot run this geometry and "
                                            "coupling");
@@ -1165,11 +1324,10 @@ struct Sim : SimBase {
         DevBuf<Step> steps;
         DevBuf<R> ckpt, traj, work, work2, mu[2], ab[2];
         DevBuf<double> gfield, gx, gy, loss, target, trace;
-        DevBuf<int> probe_of, rcnt;
+        DevBuf<int> probe_of;
         DevBuf<long long> probes;
         void release()
         {
-            rcnt.release();
             steps.release();
             ckpt.release();
             traj.release();
@@ -1214,6 +1372,10 @@ struct Sim : SimBase {
         if (trace_out)
             return trace_run(tmin, tmax, dt, events, nev, probes, nprobe,
                              trace_out, nrows, nullptr, 0);
+        /* the chain layout's adjoint runs on the grid driver (its CSR is
+         * built with it) */
+        DriverAs as_grid(driver, driver == MKH_DRIVER_CHAIN ? MKH_DRIVER_GRID
+                                                            : driver);
         if (!prog->f_adj_step)
             return fail(MKH_E_STATE, "kernel was generated without the "
                                      "adjoint");
@@ -1359,7 +1521,7 @@ struct Sim : SimBase {
         const int rty = (int)((cfg.ny + 4 * MK_ACPT - 1) / (4 * MK_ACPT));
         bool sheet_rev = false;
         if (driver == MKH_DRIVER_SHEET && prog->f_adj_sheet &&
-            sh_edges.p && (size_t)rtx * rty * 4 * 64 * 2 <= sh_edges.n &&
+            sh_edges.p && (size_t)rtx * rty * 4 * 64 * 2 * XW <= sh_edges.n &&
             (size_t)rtx * rty <= nparts) {
             int per_cu = 0;
             MKH_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -1369,7 +1531,6 @@ struct Sim : SimBase {
             sheet_rev = (long long)per_cu * prop.multiProcessorCount >=
                         (long long)rtx * rty && !no_sheet_rev;
         }
-        if (sheet_rev) MKH_HIP(adj.rcnt.alloc((size_t)rtx * rty));
         adj_rev_kind = sheet_rev ? MKH_ADJ_REV_SHEET
                        : block_rev ? MKH_ADJ_REV_BLOCK : MKH_ADJ_REV_STEP;
         adj_rev_launches = 0;
@@ -1519,12 +1680,16 @@ struct Sim : SimBase {
                 js.ab_out = abA.p;
                 MkSheetT<R> shr = sheet_args();
                 shr.tiles_y = rty;
-                shr.cnt = adj.rcnt.p;
                 shr.g0 = 0;
                 int i0 = (int)s0, i1 = (int)s1;
                 int flags = (sg == nseg - 1 ? 1 : 0) | (sg == 0 ? 2 : 0);
-                MKH_HIP(hipMemsetAsync(adj.rcnt.p, 0,
-                                       adj.rcnt.n * sizeof(int), stream));
+                /* the reverse tags start at 1 in every launch: clear the
+                 * edge words (the forward's tags are larger, and they stay
+                 * above the reverse's: tags only grow until a run's init
+                 * clears the words) */
+                MKH_HIP(hipMemsetAsync(sh_edges.p, 0,
+                                       sh_edges.n * sizeof(unsigned long long),
+                                       stream));
                 void* params[] = {&a, &js, &shr, &i0, &i1, &flags};
                 MKH_HIP(hipModuleLaunchCooperativeKernel(
                     prog->f_adj_sheet, (unsigned)(rtx * rty), 1, 1, 256, 1,
@@ -1650,6 +1815,8 @@ struct Sim : SimBase {
         if (!prog->f_adj_probe)
             return fail(MKH_E_STATE, "kernel was generated without the "
                                      "adjoint");
+        DriverAs as_grid(driver, driver == MKH_DRIVER_CHAIN ? MKH_DRIVER_GRID
+                                                            : driver);
         if (cfg.coupling == MKH_COUPLING_GRAPH && driver != MKH_DRIVER_GRID &&
             driver != MKH_DRIVER_BLOCK)
             return fail(MKH_E_STATE, "graph adjoint needs the grid or block "
@@ -2107,6 +2274,7 @@ int mkh_program_build(const char* source, const char* options,
         MKH_HIP(hipModuleGetFunction(&prog->f_block, prog->module, "mk_block"));
         MKH_HIP(hipModuleGetFunction(&prog->f_grid, prog->module, "mk_grid"));
         MKH_HIP(hipModuleGetFunction(&prog->f_sheet, prog->module, "mk_sheet"));
+        MKH_HIP(hipModuleGetFunction(&prog->f_chain, prog->module, "mk_chain"));
         MKH_HIP(hipModuleGetFunction(&prog->f_junction, prog->module,
                                      "mk_junction"));
         /* present only in kernels generated with gradients */

@@ -733,20 +733,24 @@ class SimulationOpenCL:
         return _native.COUPLING_STENCIL
 
     def set_tuning(self, driver=None, block_threads=None, sheet_cpt=None,
-                   sheet_waves=None, adjoint_waves=None):
+                   sheet_waves=None, adjoint_waves=None, chain_cpt=None,
+                   chain_waves=None):
         """Launch configuration of the device engine (chosen automatically
         by default; :func:`myokit_amd.tune.autotune` measures and picks
-        one). ``driver``: ``'auto'``, ``'cells'``, ``'block'``, ``'grid'``
-        or ``'sheet'``; ``block_threads``: threads per workgroup of the
-        cells, block and grid kernels (multiple of 64); ``sheet_cpt``:
-        cells per thread of the persistent sheet (tile 64 x 4*cpt);
-        ``sheet_waves``: waves per SIMD its register budget allows;
-        ``adjoint_waves``: the same for the reverse kernels of
-        :class:`TissueAdjoint` (0: as many as their registers allow).
-        Arguments left as None keep their current value."""
+        one). ``driver``: ``'auto'``, ``'cells'``, ``'block'``, ``'grid'``,
+        ``'sheet'`` or ``'chain'``; ``block_threads``: threads per
+        workgroup of the cells, block and grid kernels (multiple of 64);
+        ``sheet_cpt``: cells per thread of the persistent sheet (tile
+        64 x 4*cpt); ``sheet_waves``: waves per SIMD its register budget
+        allows; ``chain_cpt`` and ``chain_waves``: the same for the chain
+        driver (graphs of disjoint chains; one-wave tiles of at most
+        64*cpt cells); ``adjoint_waves``: the waves per SIMD of the reverse
+        kernels of :class:`TissueAdjoint` (0: as many as their registers
+        allow). Arguments left as None keep their current value."""
         names = {'auto': -1, 'cells': _native.DRIVER_CELLS,
                  'block': _native.DRIVER_BLOCK, 'grid': _native.DRIVER_GRID,
-                 'sheet': _native.DRIVER_SHEET}
+                 'sheet': _native.DRIVER_SHEET,
+                 'chain': _native.DRIVER_CHAIN}
         if block_threads is not None:
             block_threads = int(block_threads)
             if block_threads % 64 or not 64 <= block_threads <= 1024:
@@ -762,6 +766,15 @@ class SimulationOpenCL:
             if not 1 <= int(sheet_waves) <= 8:
                 raise ValueError('sheet_waves must be in [1, 8].')
             self._tuning['sheet_waves'] = int(sheet_waves)
+        if chain_cpt is not None:
+            if int(chain_cpt) not in _kernelgen.CHAIN_CPT_CHOICES:
+                raise ValueError('chain_cpt must be one of '
+                                 + str(_kernelgen.CHAIN_CPT_CHOICES) + '.')
+            self._tuning['chain_cpt'] = int(chain_cpt)
+        if chain_waves is not None:
+            if not 1 <= int(chain_waves) <= 8:
+                raise ValueError('chain_waves must be in [1, 8].')
+            self._tuning['chain_waves'] = int(chain_waves)
         if adjoint_waves is not None:
             if not 0 <= int(adjoint_waves) <= 8:
                 raise ValueError('adjoint_waves must be in [0, 8].')
@@ -826,6 +839,10 @@ class SimulationOpenCL:
                     'sheet_cpt', _kernelgen.sheet_cpt(self._precision)),
                 'sheet_waves': self._tuning.get(
                     'sheet_waves', _kernelgen.sheet_waves(self._precision)),
+                'chain_cpt': self._tuning.get(
+                    'chain_cpt', _kernelgen.chain_cpt(self._precision)),
+                'chain_waves': self._tuning.get(
+                    'chain_waves', _kernelgen.chain_waves(self._precision)),
                 'adjoint_waves': self._tuning.get('adjoint_waves')}
 
     def set_engine_option(self, key, value):
@@ -839,13 +856,15 @@ class SimulationOpenCL:
 
     def engine_driver(self):
         """Name of the device execution driver of the last run (``cells``,
-        ``block``, ``grid`` or ``sheet``), or None before the first run."""
+        ``block``, ``grid``, ``sheet`` or ``chain``), or None before the
+        first run."""
         if self._engine is None:
             return None
         d = ctypes.c_int(-1)
         _native.check(_native.lib().mkh_sim_driver(self._engine.h,
                                                    ctypes.byref(d)), 'driver')
-        return {0: 'cells', 1: 'block', 2: 'grid', 3: 'sheet'}[d.value]
+        return {0: 'cells', 1: 'block', 2: 'grid', 3: 'sheet',
+                4: 'chain'}[d.value]
 
     def engine_timing(self, reset=False):
         """Kernel timing of the device engine (needs the ``time_kernels``
@@ -887,7 +906,9 @@ class SimulationOpenCL:
             adjoint_precision=adjoint_precision,
             tile_cpt=tune.get('sheet_cpt'),
             tile_waves=tune.get('sheet_waves'),
-            adjoint_waves=tune.get('adjoint_waves'))
+            adjoint_waves=tune.get('adjoint_waves'),
+            chain_cells=tune.get('chain_cpt'),
+            chain_budget=tune.get('chain_waves'))
         if len(self._sources) >= 16:
             self._sources.pop(next(iter(self._sources)))
         self._sources[gkey] = (src, cm)
@@ -953,6 +974,8 @@ class SimulationOpenCL:
         cpt = tune.get('sheet_cpt') or _kernelgen.sheet_cpt(self._precision)
         cfg.sheet_cpt = cpt if _kernelgen.sheet_enabled(self._precision) \
             else 0
+        cfg.chain_cpt = tune.get('chain_cpt') or \
+            _kernelgen.chain_cpt(self._precision)
         eng = _Engine(prog, cfg)
         eng.options = {}
         eng.config_version = None

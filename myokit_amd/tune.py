@@ -10,10 +10,12 @@ per model, geometry, precision, coupling and device, so later simulations
 of the same kind can :func:`apply_cached` without measuring.
 
 Candidates: the drivers that can run the geometry (uncoupled cells; one
-workgroup per domain; one step per launch; the persistent sheet) crossed
-with threads per workgroup (128, 256, 512) for the first three, and cells
-per thread (tile heights 8, 16, 32) and register budgets (waves per SIMD)
-for the sheet. A candidate that fails to build or launch is skipped.
+workgroup per domain; one step per launch; the persistent sheet; the
+persistent chains) crossed with threads per workgroup (128, 256, 512, 1024)
+for the first three, cells per thread (tile heights 8, 16, 32) and register
+budgets (waves per SIMD) for the sheet, and cells per thread and register
+budgets for the chains. A candidate that fails to build or launch is
+skipped.
 """
 import hashlib
 import json
@@ -60,7 +62,7 @@ def candidates(sim):
     if c == _native.COUPLING_NONE:
         drivers = ['cells', 'grid']
     elif c == _native.COUPLING_GRAPH:
-        drivers = ['block', 'grid']
+        drivers = ['block', 'grid', 'chain']
     else:
         drivers = ['block', 'grid']
         if c == _native.COUPLING_STENCIL and sim._ny > 1:
@@ -73,6 +75,12 @@ def candidates(sim):
                 for w in waves:
                     out.append({'driver': 'sheet', 'block_threads': 256,
                                 'sheet_cpt': cpt, 'sheet_waves': w})
+        elif d == 'chain':
+            waves = (2, 3) if sim._precision == 64 else (4, 5, 6)
+            for cpt in (1, 2):
+                for w in waves:
+                    out.append({'driver': 'chain', 'chain_cpt': cpt,
+                                'chain_waves': w})
         elif d == 'block' and n <= 64:
             out.append({'driver': 'block', 'block_threads': 64})
         else:

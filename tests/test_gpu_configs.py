@@ -124,7 +124,8 @@ def test_config3_uneven_cables_reduced(precision):
     """Config 3(ii) reduced to 12 cables with lengths drawn uniformly in
     [50, 400] (seeded) and 60 ms: chains built with set_connections
     (g = 9.5), the first 5 cells of each cable paced, FE, dt 0.005 ms, all
-    cables in one launch (one workgroup per cable)."""
+    cables in one launch (the chain driver: one-wave tiles laid end to
+    end)."""
     rng = np.random.default_rng(11)
     lengths = rng.integers(50, 401, 12)
     conns, paced = [], []
@@ -136,7 +137,7 @@ def test_config3_uneven_cables_reduced(precision):
     sc = Scenario('lr1991', int(off), dt=0.005, connections=conns,
                   paced_list=paced, events=[(1, 0, 2, 1000, 0)])
     P, O = run_both(sc, 60.0, precision, log_interval=0.5)
-    assert P['sim'].engine_driver() == 'block'
+    assert P['sim'].engine_driver() == 'chain'
     if precision == 64:
         assert _fp64_report('config3 cables', P, O) <= FP64_TOL
     else:

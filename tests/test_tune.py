@@ -28,7 +28,11 @@ def test_candidates_by_coupling():
     assert {x['driver'] for x in c} == {'cells', 'grid'}
     s = sim(10)
     s.set_connections([(i, i + 1, 1.0) for i in range(9)])
-    assert {x['driver'] for x in tune.candidates(s)} == {'block', 'grid'}
+    c = tune.candidates(s)
+    assert {x['driver'] for x in c} == {'block', 'grid', 'chain'}
+    from myokit_amd import _kernelgen
+    assert all(x['chain_cpt'] in _kernelgen.CHAIN_CPT_CHOICES
+               for x in c if x['driver'] == 'chain')
     for x in tune.candidates(sim((64, 64), precision=64)):
         if x['driver'] == 'sheet':
             assert x['sheet_cpt'] in (2, 4)
@@ -52,11 +56,16 @@ def test_set_tuning_validation():
         s.set_tuning(driver='fast')
     with pytest.raises(ValueError):
         s.set_tuning(adjoint_waves=9)
+    with pytest.raises(ValueError):
+        s.set_tuning(chain_cpt=3)
+    with pytest.raises(ValueError):
+        s.set_tuning(chain_waves=9)
     s.set_tuning(driver='grid', block_threads=128, sheet_cpt=8,
-                 sheet_waves=3, adjoint_waves=4)
+                 sheet_waves=3, adjoint_waves=4, chain_cpt=1, chain_waves=6)
     t = s.tuning()
     assert t == {'driver': 'grid', 'block_threads': 128, 'sheet_cpt': 8,
-                 'sheet_waves': 3, 'adjoint_waves': 4}
+                 'sheet_waves': 3, 'adjoint_waves': 4, 'chain_cpt': 1,
+                 'chain_waves': 6}
 
 
 def test_cache_round_trip(tmp_path, monkeypatch):
