@@ -29,15 +29,24 @@ CLOCK_GHZ = 2.4              # MI355X peak engine clock
 SIMDS = 256 * 4              # 256 CUs x 4 SIMDs
 
 # mk_sheet on this workload (FP32), from rocprofv3 counters of 200-step
-# dispatches (profiles/mk_sheet_lr1991_1024x1024_fp32.md):
-# - HBM traffic per cell-step: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors of
-#   a 256 MiB device copy), 307.6 + 340.8 MiB per 2.10e8 cell-steps;
-# - per wave-step of 4 cells: 840 VALU instructions at 4 cycles and 224
-#   transcendental ones (quarter rate) at 16.
-SHEET_TRAFFIC_B_PER_CELL_STEP = (307.6 + 340.8) * 2.0**20 / (200 * 2.0**20)
-SHEET_VALU = 840
+# dispatches taken at commit 4c02761
+# (profiles/mk_sheet_lr1991_1024x1024_fp32.md):
+# - HBM traffic: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors of a 256 MiB
+#   device copy, bench.py --copy-calibration), 182,520 x 2 + 444,420 KiB per
+#   200 x 2^20 cell-steps;
+# - per wave-step of 4 cells: 1,105 VALU instructions, 224 of them
+#   transcendental; a transcendental holds the SIMD 8 cycles, any other VALU
+#   instruction 4 (SQ_ACTIVE_INST_VALU = VALU - TRANS + 2 TRANS);
+# - VALU busy (SQ_ACTIVE_INST_VALU x 4 / SIMDs / cycles per XCD): 0.77.
+SHEET_PROFILE = {'file': 'profiles/mk_sheet_lr1991_1024x1024_fp32.md',
+                 'commit': '4c02761'}
+SHEET_TRAFFIC_B_PER_CELL_STEP = ((182520 * 2 + 444420) * 1024.0
+                                 / (200 * 2.0**20))
+SHEET_VALU = 1105
 SHEET_TRANS = 224
-SHEET_VALU_CYCLES_PER_WAVE_STEP = SHEET_VALU * 4 + SHEET_TRANS * 16
+SHEET_VALU_CYCLES_PER_WAVE_STEP = (SHEET_VALU - SHEET_TRANS) * 4 \
+    + SHEET_TRANS * 8
+SHEET_VALU_BUSY = 0.77
 SHEET_CELLS_PER_LANE = 4
 
 
@@ -477,7 +486,16 @@ def main():
                     help='adjoint workloads: simulated window')
     ap.add_argument('--sets', type=int, default=32,
                     help='graph-adjoint: parameter sets in the batch')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only fill 256 MiB and copy it device to device 4 '
+                    'times (known traffic per dispatch, which scales the '
+                    'HBM counters of a profile) and exit')
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_bandwidth(nbytes=256 << 20, reps=3)
+        print('copy done: 1 fill and 4 copies of 256 MiB', flush=True)
+        return
 
     if args.gpus != 1:
         print('bench.py runs on one GPU only: the engine has no multi-GPU'
@@ -523,8 +541,10 @@ def main():
     copy_gbs = copy_bandwidth()
 
     line = {
-        'metric': 'cell-steps/s (2D 1024² LR1991 fp32 fwd); '
-                  'fwd+adjoint vs fwd time ratio',
+        'metric': 'cell-steps/s (2D %s LR1991 fp%d fwd); '
+                  'fwd+adjoint vs fwd time ratio' % (
+                      '1024²' if (args.nx, args.ny) == (1024, 1024)
+                      else '%dx%d' % (args.nx, args.ny), args.precision),
         'value': value,
         'unit': 'cell-steps/s',
         'n_gpus': 1,
@@ -560,6 +580,12 @@ def main():
             'frac_of_copy': achieved / copy_gbs,
         },
     }
+    line['roofline']['achieved_is'] = (
+        'algorithmic-equivalent bandwidth: %d B per cell-step (the state '
+        'read and written once per time step) over the launch time; the '
+        'persistent drivers keep the state in registers for a whole launch, '
+        'so the HBM traffic they cause is roofline.traffic' % (
+            bytes_per_cell_step))
     if (args.precision == 32 and s.engine_driver() == 'sheet'
             and (args.nx, args.ny) == (1024, 1024)):
         # Counter-derived figures of this kernel on this workload: HBM bytes
@@ -569,16 +595,19 @@ def main():
         model_us = (waves / SIMDS * SHEET_VALU_CYCLES_PER_WAVE_STEP
                     / (CLOCK_GHZ * 1e3))
         meas_us = avg_ms * 1e3 / steps_per_launch
-        line['roofline']['traffic'] = (SHEET_TRAFFIC_B_PER_CELL_STEP
-                                       * cells_per_launch)
+        traffic = SHEET_TRAFFIC_B_PER_CELL_STEP * cells_per_launch
+        line['roofline']['traffic'] = traffic
+        line['roofline']['traffic_gbs'] = traffic / (avg_ms * 1e-3) / 1e9
         line['roofline']['valu'] = {
             'bound_us_per_time_step': model_us,
             'measured_us_per_time_step': meas_us,
             'frac': model_us / meas_us,
-            'model': '%d VALU x 4 + %d transcendental x 16 cycles per '
-                     'wave-step of 4 cells, 4 SIMDs x 256 CUs, 2.4 GHz '
-                     '(rocprofv3 SQ counters)' % (SHEET_VALU, SHEET_TRANS),
+            'busy_counter': SHEET_VALU_BUSY,
+            'model': '(%d - %d) VALU x 4 + %d transcendental x 8 cycles per '
+                     'wave-step of 4 cells, 4 SIMDs x 256 CUs, 2.4 GHz' % (
+                         SHEET_VALU, SHEET_TRANS, SHEET_TRANS),
         }
+        line['roofline']['profile'] = SHEET_PROFILE
     if args.adjoint_ms > 0:
         fwd_ms = el / args.steps * 1e3 * args.adjoint_ms / seg
         n_adj, res = adjoint_leg(s, args.adjoint_ms, [0, 200],

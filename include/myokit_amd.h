@@ -78,6 +78,9 @@ typedef struct mkh_sim_config {
                               the kernel source), 0 = no sheet driver */
     int chain_cpt;         /* cells per thread of the chain driver (MK_CCPT in
                               the kernel source), 0 = no chain driver */
+    int adj_cpt;           /* cells per thread of the persistent reverse sheet
+                              (MK_ACPT in the kernel source: tiles of
+                              64 x 4 adj_cpt cells), 0 = no reverse sheet */
 } mkh_sim_config;
 
 /* Library */

@@ -594,45 +594,169 @@ def generate_cell_function(cm, precision, native_maths=False):
     sweep = forward_sweep(cm, w, tab)
     ns = len(cm.states)
     lines.extend(sweep[:ns] + [tab + x for x in fam] + sweep[ns:])
-    rl = cm.rl_states
     if cm.quantities:
         lines.append(tab + 'if (LOG) {')
         for k, var in enumerate(cm.quantities):
             lines.append(2 * tab + 'q[' + str(k) + '] = '
                          + w.ex(ex.Name(var)) + ';')
         lines.append(tab + '}')
-    expf = w.fn('exp')
     for i, s in enumerate(cm.states):
-        x = _cname(s, 'x_')
-        if s in rl:
-            inf, tau = rl[s]
-            inf, tau = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
-            # RL update x = inf - (inf - x) * exp(-dt / tau), as written by
-            # the reference (openclsim.cl:358-364)
-            trhs = rl[s][1].rhs()
-            if precision == DOUBLE_PRECISION:
-                E = expf + '(-dt / ' + tau + ')'
-            else:
-                if isinstance(trhs, ex.Divide) and isinstance(
-                        trhs[0], ex.Number) and trhs[0].eval() == 1:
-                    # tau = 1 / X (the form the RL conversion writes), so
-                    # -dt / tau = -dt * X without two reciprocals
-                    rate = w.ex(trhs[1])
-                else:
-                    rate = '__builtin_amdgcn_rcpf(' + tau + ')'
-                # exp(-dt X) = 2^((-dt log2(e)) X): the first factor is the
-                # same for every cell of a step, so one multiply and one
-                # v_exp_f32 per gate; within 3 ulp of exp (OpenCL's bound)
-                # for dt X <= 1, and below 1e-8 in absolute terms beyond
-                E = ('__builtin_amdgcn_exp2f((-dt * 0x1.715476p+0f) * '
-                     + rate + ')')
-            lines.append(tab + 'y[' + str(i) + '] = ' + inf + ' - (' + inf
-                         + ' - ' + x + ') * ' + E + ';')
-        else:
-            lines.append(tab + 'y[' + str(i) + '] = ' + x + ' + dt * '
-                         + _cname(s, 'd_') + ';')
+        lines.append(tab + _state_update(cm, w, precision, i, s))
     lines.append('}')
     return '\n'.join(lines) + '\n'
+
+
+def _state_update(cm, w, precision, i, s):
+    """The C statement that advances state ``s`` (index ``i``) by one step:
+    y[i] = x + dt * dot(x), or the Rush-Larsen update."""
+    x = _cname(s, 'x_')
+    rl = cm.rl_states
+    if s not in rl:
+        return 'y[' + str(i) + '] = ' + x + ' + dt * ' + _cname(s, 'd_') + ';'
+    inf, tau = rl[s]
+    inf, tau = w.ex(ex.Name(inf)), w.ex(ex.Name(tau))
+    # RL update x = inf - (inf - x) * exp(-dt / tau), as written by the
+    # reference (openclsim.cl:358-364)
+    trhs = rl[s][1].rhs()
+    if precision == DOUBLE_PRECISION:
+        E = w.fn('exp') + '(-dt / ' + tau + ')'
+    else:
+        if isinstance(trhs, ex.Divide) and isinstance(
+                trhs[0], ex.Number) and trhs[0].eval() == 1:
+            # tau = 1 / X (the form the RL conversion writes), so
+            # -dt / tau = -dt * X without two reciprocals
+            rate = w.ex(trhs[1])
+        else:
+            rate = '__builtin_amdgcn_rcpf(' + tau + ')'
+        # exp(-dt X) = 2^((-dt log2(e)) X): the first factor is the same
+        # for every cell of a step, so one multiply and one v_exp_f32 per
+        # gate; within 3 ulp of exp (OpenCL's bound) for dt X <= 1, and
+        # below 1e-8 in absolute terms beyond
+        E = ('__builtin_amdgcn_exp2f((-dt * 0x1.715476p+0f) * ' + rate
+             + ')')
+    return ('y[' + str(i) + '] = ' + inf + ' - (' + inf + ' - ' + x + ') * '
+            + E + ';')
+
+
+def diffusion_dependents(cm):
+    """(variables, states): the intermediaries whose value depends on the
+    diffusion current (with the variable bound to it), and the states whose
+    derivative does. Everything else of a cell step can be evaluated before
+    the neighbours' potentials of the step are known."""
+    dep = set(v for v, what in cm.bound.items() if what == 'idiff')
+    dstates = set()
+    for var in cm.order:
+        hit = False
+        for r in var.rhs().references():
+            v = r.var()
+            if isinstance(r, ex.Derivative):
+                hit = hit or v in dstates
+            else:
+                hit = hit or v in dep
+        if hit:
+            (dstates if var.is_state() else dep).add(var)
+    return dep, dstates
+
+
+def generate_split_cell_function(cm, precision, native_maths=False):
+    """Returns the HIP source of ``mk_cell_a<LOG>`` and ``mk_cell_b<LOG>``,
+    the cell step of ``mk_cell`` in two parts with the same operations:
+    part A evaluates every equation, logged quantity and state update that
+    does not depend on the diffusion current, and keeps what part B reads
+    in ``carry`` (``MK_NCARRY`` values); part B, given the diffusion
+    current, evaluates the rest. A persistent driver runs part A of all its
+    cells before it waits for the neighbours' potentials of the step, and
+    part B after. ``mk_qpart[k]`` says which part sets logged quantity k."""
+    import re
+    w = cell_writer(cm, precision, native_maths)
+    tab = '    '
+    dep, dstates = diffusion_dependents(cm)
+    fam = []
+    if precision == SINGLE_PRECISION:
+        fam, members = exp_families(cm, w)
+        w.set_exp_members(members)
+    a_lines, b_lines = [], []
+    declared = []
+    for i, s in enumerate(cm.states):
+        a_lines.append('const Real ' + _cname(s, 'x_') + ' = y[' + str(i)
+                       + '];')
+        declared.append(_cname(s, 'x_'))
+    for var in cm.per_cell_constants:
+        a_lines.append('const Real ' + _cname(var, 'c_') + ' = '
+                       + w.ex(var.rhs()) + ';')
+        declared.append(_cname(var, 'c_'))
+    for line in fam:
+        a_lines.append(line)
+        declared.append(line.split()[2])
+    for var in cm.order:
+        if var.is_state():
+            if var in cm.rl_states:
+                continue
+            name, into = _cname(var, 'd_'), (b_lines if var in dstates
+                                               else a_lines)
+        else:
+            name, into = _cname(var), (b_lines if var in dep else a_lines)
+        line = 'const Real ' + name + ' = ' + w.ex(var.rhs()) + ';'
+        into.append(line)
+        if into is a_lines:
+            declared.append(name)
+    qa, qb, qpart = [], [], []
+    for k, var in enumerate(cm.quantities):
+        line = 'q[' + str(k) + '] = ' + w.ex(ex.Name(var)) + ';'
+        in_b = var in dep
+        (qb if in_b else qa).append(line)
+        qpart.append(1 if in_b else 0)
+    ua, ub = [], []
+    for i, s in enumerate(cm.states):
+        if s in cm.rl_states:
+            inf, tau = cm.rl_states[s]
+            in_b = inf in dep or tau in dep
+        else:
+            in_b = s in dstates
+        (ub if in_b else ua).append(_state_update(cm, w, precision, i, s))
+    # what part B reads of part A's values
+    b_text = '\n'.join(b_lines + qb + ub)
+    used = set(re.findall(r'[A-Za-z_][A-Za-z0-9_]*', b_text))
+    carry = [n for n in declared if n in used]
+    out = ['#define MK_NCARRY ' + str(max(1, len(carry))),
+           '__device__ constexpr int mk_qpart[' + str(max(1, len(qpart)))
+           + '] = {' + ', '.join(str(x) for x in (qpart or [0])) + '};']
+    sig = ('Real* __restrict__ y, const Real t, const Real dt, '
+           'const Real pace,\n' + tab)
+    out.append('template <bool LOG>')
+    out.append('__device__ __forceinline__ void mk_cell_a(')
+    out.append(tab + sig + 'const Real* __restrict__ fv, '
+               'Real* __restrict__ q, Real* __restrict__ carry)')
+    out.append('{')
+    out.append(tab + '(void)t; (void)dt; (void)pace; (void)fv; (void)q;'
+               ' (void)carry;')
+    out += [tab + x for x in a_lines]
+    if qa:
+        out.append(tab + 'if (LOG) {')
+        out += [2 * tab + x for x in qa]
+        out.append(tab + '}')
+    out += [tab + x for x in ua]
+    out += [tab + 'carry[' + str(j) + '] = ' + n + ';'
+            for j, n in enumerate(carry)]
+    out.append('}')
+    out.append('template <bool LOG>')
+    out.append('__device__ __forceinline__ void mk_cell_b(')
+    out.append(tab + sig + 'const Real idiff, const Real* __restrict__ fv, '
+               'Real* __restrict__ q,\n' + tab
+               + 'const Real* __restrict__ carry)')
+    out.append('{')
+    out.append(tab + '(void)t; (void)dt; (void)pace; (void)idiff; (void)fv;'
+               ' (void)q; (void)carry;')
+    out += [tab + 'const Real ' + n + ' = carry[' + str(j) + '];'
+            for j, n in enumerate(carry)]
+    out += [tab + x for x in b_lines]
+    if qb:
+        out.append(tab + 'if (LOG) {')
+        out += [2 * tab + x for x in qb]
+        out.append(tab + '}')
+    out += [tab + x for x in ub]
+    out.append('}')
+    return '\n'.join(out) + '\n'
 
 
 _FLOAT_HELPERS = '''
@@ -669,20 +793,32 @@ __device__ __forceinline__ float mk_powif(float x, int n)
 '''
 
 
-# Cells per thread of the persistent reverse sheet, as MK_ACPT in
-# csrc/mk_args.h (which the host library reads too)
-ADJ_SHEET_CPT = 8
+# Cells per thread of the persistent reverse sheet (MK_ACPT in the kernel
+# source; the host library reads it from mkh_sim_config.adj_cpt)
+ADJ_SHEET_CPT_CHOICES = (2, 4, 5, 6, 8)
 
 
-def adjoint_sheet_groups(cm, precision, cpt=ADJ_SHEET_CPT):
+def adjoint_sheet_cpt(precision):
+    """Cells per thread of the persistent reverse sheet (tiles of
+    64 x 4 cpt cells). Every tile is co-resident, so a CU holds its share of
+    the sheet (4,096 cells of 1024^2) and mu of those cells in LDS (32 B
+    per cell in single precision) whatever the tile size; the tile size
+    only sets how many workgroups share it. 8: two workgroups of 73 KiB
+    per CU, and the LR1991 reverse step takes 241 registers per lane
+    without spills. 4 would give four workgroups per CU (four waves per
+    SIMD) but caps the step at 128 registers, where it spills 68; 5 gives
+    three per CU, whose 768 tiles per device are fewer than the 832 a
+    1024^2 sheet needs."""
+    return 8
+
+
+def adjoint_sheet_groups(cm, precision, cpt=None):
     """Workgroups per CU (at most 4) that the LDS of the persistent reverse
     sheet leaves room for: mk_adj_sheet keeps mu of its 64 x 4 cpt tile and
     abar with a one-cell ring in LDS. 0 if one does not fit into a CU's
-    160 KiB. With 8 cells per thread (64 x 32 tiles) an LR1991 tile takes
-    73 KiB in single precision: two per CU, 254 registers per lane and no
-    spills; 4 cells per thread would allow four per CU but spill at the
-    128 registers that leaves, which costs more than the occupancy
-    gains."""
+    160 KiB."""
+    if cpt is None:
+        cpt = adjoint_sheet_cpt(precision)
     w = 8 if precision == DOUBLE_PRECISION else 4
     th = 4 * cpt
     lds = len(cm.states) * th * 64 * w + (th + 2) * 66 * w + 64 + 4
@@ -722,8 +858,13 @@ def sheet_enabled(precision):
 
 def chain_cpt(precision):
     """Cells per thread of the persistent chain driver: one-wave tiles of up
-    to 64 * cpt cells."""
-    return 2
+    to 64 * cpt cells. On 256 LR1991 cables of 50-2000 cells (256,157
+    cells), single precision runs at 5.06e10 / 5.87e10 / 5.75e10
+    cell-steps/s with 1 / 2 / 4 cells per thread and double precision at
+    1.14e10 / 1.00e10 / 0.66e10 (the double-precision step holds twice the
+    registers per cell, so one cell per thread leaves room for more
+    waves)."""
+    return 1 if precision == DOUBLE_PRECISION else 2
 
 
 def chain_waves(precision):
@@ -742,7 +883,8 @@ SHEET_CPT_CHOICES = (1, 2, 4, 8)
 def generate_kernel(cm, precision, coupling, block_threads=256,
                     native_maths=False, adjoint=False, tile_cpt=None,
                     tile_waves=None, adjoint_precision=None,
-                    adjoint_waves=None, chain_cells=None, chain_budget=None):
+                    adjoint_waves=None, chain_cells=None, chain_budget=None,
+                    adjoint_cpt=None):
     """Returns the complete HIP source for hiprtc. ``tile_cpt`` (cells
     per thread of the persistent sheet: tile height 4 x cpt) and
     ``tile_waves`` (waves per SIMD its register budget allows) default to
@@ -753,7 +895,8 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
     budget allows (0: no limit; default: :func:`adjoint_waves`).
     ``chain_cells`` and ``chain_budget`` set the chain driver's cells per
     thread and waves per SIMD (defaults :func:`chain_cpt`,
-    :func:`chain_waves`)."""
+    :func:`chain_waves`), ``adjoint_cpt`` the persistent reverse sheet's
+    cells per thread (default :func:`adjoint_sheet_cpt`)."""
     if block_threads % 64 or not 64 <= block_threads <= 1024:
         raise ValueError('block_threads must be a multiple of 64 in'
                          ' [64, 1024].')
@@ -777,7 +920,12 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
     i_vm = vm.index() if (vm is not None and vm.is_state()) else 0
     bcells = 2048 if precision == DOUBLE_PRECISION else 4096
     real = 'double' if precision == DOUBLE_PRECISION else 'float'
-    asg = adjoint_sheet_groups(cm, precision)
+    acpt = adjoint_sheet_cpt(precision) if adjoint_cpt is None \
+        else int(adjoint_cpt)
+    if acpt not in ADJ_SHEET_CPT_CHOICES:
+        raise ValueError('adjoint_cpt must be one of '
+                         + str(ADJ_SHEET_CPT_CHOICES) + '.')
+    asg = adjoint_sheet_groups(cm, precision, acpt)
     parts = [
         '// Generated by myokit_amd for model: '
         + str(cm.model.name()).replace('\n', ' '),
@@ -792,6 +940,7 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_SWAVES ' + str(waves),
         '#define MK_CCPT ' + str(ccpt),
         '#define MK_CWAVES ' + str(cwaves),
+        '#define MK_ACPT ' + str(acpt),
         '#define MK_ADJ_SHEET ' + ('1' if adjoint and int(coupling) == 1
                                    and asg > 0
                                    and adjoint_precision in (None, precision)
@@ -805,6 +954,7 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         _read('mk_args.h'),
         _FLOAT_HELPERS,
         generate_cell_function(cm, precision, native_maths),
+        generate_split_cell_function(cm, precision, native_maths),
         _read('mk_drivers.hip'),
     ]
     if adjoint:

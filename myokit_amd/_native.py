@@ -47,7 +47,7 @@ class SimConfig(ctypes.Structure):
         ('precision', _i), ('n_state', _i), ('n_field', _i),
         ('n_quant', _i), ('i_vm', _i), ('coupling', _i), ('driver', _i),
         ('block_threads', _i), ('nx', _ll), ('ny', _ll), ('device', _i),
-        ('sheet_cpt', _i), ('chain_cpt', _i),
+        ('sheet_cpt', _i), ('chain_cpt', _i), ('adj_cpt', _i),
     ]
 
 

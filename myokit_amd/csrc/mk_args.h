@@ -74,9 +74,6 @@ struct MkAdjT {
     int target_row;         /* target row compared with V(y_{s+1}), or -1 */
 };
 
-/* Cells per thread of the persistent reverse sheet (mk_adj_sheet): a tile
- * of 64 x (4 * MK_ACPT) cells per 256-thread workgroup. */
-#define MK_ACPT 8
 
 /* Persistent sheet driver: neighbour exchange between workgroup tiles.
  * An exchanged value is one 64-bit word per 32 bits of the value, each word

@@ -30,7 +30,7 @@
  *   mk_sheet  large homogeneous 2D sheets: a persistent cooperative launch,
  *             one 64 x 16 tile per workgroup with its state in registers
  *             for every step of the launch; tile edges are exchanged
- *             through L2 with a step counter per tile;
+ *             through L2 as step-tagged words;
  *   mk_chain  graphs of disjoint chains (cables): the same persistent
  *             scheme over the cells laid end to end, a cable spread over
  *             as many tiles as its length needs.
@@ -43,6 +43,13 @@
  */
 typedef MkArgsT<Real> MkArgs;
 typedef MkStepT<Real> MkStep;
+
+/* The layout this text was generated for: precision (bits), states,
+ * fields, and the cells per thread of the sheet, chain and reverse-sheet
+ * tiles. The host library reads it when it loads the module and rejects a
+ * simulation configured for another layout. */
+extern "C" __device__ const int mk_layout[6] = {
+    (int)(sizeof(Real) * 8), MK_NS, MK_NF, MK_SCPT, MK_CCPT, MK_ACPT};
 
 #define MK_NFA (MK_NF > 0 ? MK_NF : 1)
 #define MK_NQA (MK_NQ > 0 ? MK_NQ : 1)
@@ -147,6 +154,20 @@ __device__ __forceinline__ void mk_write_log(
     const MkArgs& a, long long c, int row, const Real* q)
 {
     for (int k = 0; k < MK_NQ; ++k) {
+        const int col = a.colmap[(long long)k * a.npad + c];
+        if (col >= 0) a.logbuf[(long long)row * a.ncols + col] = q[k];
+    }
+}
+
+/* The logged quantities set by part P of the split cell step (mk_cell_a:
+ * 0, mk_cell_b: 1). */
+template <int P>
+__device__ __forceinline__ void mk_write_log_part(
+    const MkArgs& a, long long c, int row, const Real* q)
+{
+#pragma unroll
+    for (int k = 0; k < MK_NQ; ++k) {
+        if (mk_qpart[k] != P) continue;
         const int col = a.colmap[(long long)k * a.npad + c];
         if (col >= 0) a.logbuf[(long long)row * a.ncols + col] = q[k];
     }
@@ -354,11 +375,19 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
  * the full state of cells (x0 + lx, y0 + w * MK_SCPT + k), k < MK_SCPT, in
  * registers. The tile's membrane potentials, plus a one-cell halo ring, live
  * in LDS (double-buffered by step). After each step a tile writes its four
- * edge rows/columns of V to global memory as step-tagged words; before the
- * next step each thread of its halo ring waits for its neighbour value of
- * that step and copies it into LDS (one barrier per step). No state goes to
- * HBM inside a launch: per cell-step the only global traffic is the edge
+ * edge rows/columns of V to global memory as step-tagged words; in the next
+ * step each thread of its halo ring waits for its neighbour value of that
+ * step and copies it into LDS (one barrier per step). No state goes to HBM
+ * inside a launch: per cell-step the only global traffic is the edge
  * exchange.
+ *
+ * A step runs the generated cell step in two parts (mk_cell_a, mk_cell_b):
+ * part A of all the thread's cells first, everything that does not depend
+ * on the diffusion current (for LR1991 all but the update of V), with what
+ * part B reads kept in LDS; then the halo wait and the barrier; then each
+ * cell's diffusion current and part B. The neighbours' edge words travel
+ * while part A computes, so the wait rarely finds them missing. The
+ * operations, and so the results, are those of mk_cell.
  *
  * The launch is cooperative, so all tiles are resident at once. Every wait
  * is bounded (spin_limit polls); on a timeout the tile records an error and
@@ -444,6 +473,10 @@ extern "C" __global__ void __launch_bounds__(256, MK_SWAVES)
 mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
 {
     __shared__ Real vt[2][MK_STH + 2][66];
+    /* what part B of each cell's step reads of part A's values: in LDS
+     * rather than registers, where it would add to the live state of all
+     * the thread's cells across the halo wait */
+    __shared__ Real ca[MK_NCARRY][MK_SCPT][256];
     __shared__ int wg_flag;
     if (threadIdx.x == 0)
         wg_flag = mk_halted(a, a.steps[s1 - 1].row) ? 1 : 0;
@@ -462,12 +495,18 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
     Real q[MK_NQA];
     bool act[MK_SCPT];
     bool paced[MK_SCPT];
+    /* cell indices in 32 bits (a co-resident sheet has far fewer than 2^31
+     * cells) and whether the NaN test reads the cell */
+    int cid[MK_SCPT];
+    bool nanc[MK_SCPT];
 #pragma unroll
     for (int k = 0; k < MK_SCPT; ++k) {
         const int r = w * MK_SCPT + k;
         const long long iy = gy0 + r;
         act[k] = ix < nx && iy < ny;
         const long long c = act[k] ? ix + iy * nx : 0;
+        cid[k] = (int)c;
+        nanc[k] = act[k] && a.nan_check && c == a.nan_cell;
         paced[k] = act[k] && mk_is_paced(a, c, ix, iy);
 #pragma unroll
         for (int i = 0; i < MK_NS; ++i)
@@ -508,13 +547,39 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
     int cur = 0;
     for (int s = s0; s < s1; ++s) {
         const int i = s - s0;
+        const MkStep p = a.steps[s];
+        const bool last = s + 1 == s1;
+        const int par = (sh.g0 + i) & 1;
+        /* Part A of every cell's step: all that does not depend on the
+         * diffusion current, so the neighbours' edges of the previous step
+         * are on their way meanwhile */
+#pragma unroll
+        for (int k = 0; k < MK_SCPT; ++k) {
+            if (act[k]) {
+                if (p.row >= 0 && nanc[k] && y[k][0] != y[k][0])
+                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+                const Real pace = paced[k] ? p.pace : (Real)0;
+                Real cy[MK_NCARRY];
+                /* logged values are register copies, written out only at
+                 * log steps */
+                mk_cell_a<true>(y[k], p.t, p.dt, pace, fv[k], q, cy);
+#pragma unroll
+                for (int j = 0; j < MK_NCARRY; ++j)
+                    ca[j][k][threadIdx.x] = cy[j];
+                if (p.row >= 0) mk_write_log_part<0>(a, cid[k], p.row, q);
+            }
+            /* one cell after the other: interleaving the cells' part A
+             * buys no latency the other waves do not hide, and multiplies
+             * the live registers */
+            __builtin_amdgcn_sched_barrier(0);
+        }
         if (i > 0) {
             /* The neighbours' edges of the previous step (south: its top
              * row, north: its bottom row, west: its right column, east: its
              * left column) into the halo ring, each halo thread waiting for
              * its own value */
             const int step = sh.g0 + i;
-            const int par = (step - 1) & 1;
+            const int hpar = (step - 1) & 1;
             const int t = threadIdx.x;
             int nbt = -1, e = 0, k = 0, lr = 0, lc = 0;
             if (t < 64) {
@@ -533,8 +598,8 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
              * not */
             if (nbt >= 0 && (e < 2 ? gx0 + k < nx : gy0 + k < ny)) {
                 Real v;
-                if (mk_tget(mk_edge(sh, par, nbt, e, k), step, sh.spin_limit,
-                            v)) {
+                if (mk_tget(mk_edge(sh, hpar, nbt, e, k), step,
+                            sh.spin_limit, v)) {
                     vt[cur][lr][lc] = v;
                 } else {
                     __hip_atomic_store(sh.err, 1, __ATOMIC_RELAXED,
@@ -545,9 +610,8 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
             __syncthreads();
             if (wg_flag) return;
         }
-        const MkStep p = a.steps[s];
-        const bool last = s + 1 == s1;
-        const int par = (sh.g0 + i) & 1;
+        /* Part B: the diffusion current from the potentials at t, then the
+         * rest of the step */
 #pragma unroll
         for (int k = 0; k < MK_SCPT; ++k) {
             const int r = w * MK_SCPT + k;
@@ -570,14 +634,13 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
                                        - vt[cur][r + 2][lx + 1]);
                 }
                 const Real pace = paced[k] ? p.pace : (Real)0;
-                const long long c = ix + iy * nx;
-                if (p.row >= 0 && a.nan_check && c == a.nan_cell &&
-                    y[k][0] != y[k][0])
-                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
-                /* One instance of the cell code: logged values are register
-                 * copies, written out only at log steps */
-                mk_cell<true>(y[k], p.t, p.dt, pace, id, fv[k], q);
-                if (p.row >= 0) mk_write_log(a, c, p.row, q);
+                const long long c = cid[k];
+                Real cy[MK_NCARRY];
+#pragma unroll
+                for (int j = 0; j < MK_NCARRY; ++j)
+                    cy[j] = ca[j][k][threadIdx.x];
+                mk_cell_b<true>(y[k], p.t, p.dt, pace, id, fv[k], q, cy);
+                if (p.row >= 0) mk_write_log_part<1>(a, c, p.row, q);
                 if (sh.traj) mk_store_traj(a, sh.traj, i, c, y[k]);
                 const Real vn = y[k][MK_IVM];
                 vt[cur ^ 1][r + 1][lx + 1] = vn;
@@ -625,8 +688,10 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
  * several: after each step a tile stores its first and last V as
  * step-tagged words (the exchange of mk_sheet); before the next step it
  * waits (bounded) for the value of each neighbour it is connected to. Tiles
- * whose boundary falls between two cables neither wait nor read there. Where the block driver gives a
- * whole cable to one workgroup, so that the longest cable sets the
+ * whose boundary falls between two cables neither wait nor read there. As
+ * in mk_sheet, part A of the cells' step runs before the halo wait and
+ * part B (with the diffusion current) after it. Where the block driver
+ * gives a whole cable to one workgroup, so that the longest cable sets the
  * launch's duration, here every cable is spread over as many tiles as its
  * length needs.
  *
@@ -704,17 +769,41 @@ mk_chain(const MkArgs a, const MkChain ch, const int s0, const int s1)
     __syncthreads();
 
     int cur = 0;
+    Real cy[MK_CCPT][MK_NCARRY];
     for (int s = s0; s < s1; ++s) {
         const int i = s - s0;
+        const MkStep p = a.steps[s];
+        const bool last = s + 1 == s1;
+        const int par = (ch.g0 + i) & 1;
+        /* Part A of every cell's step (all that does not depend on the
+         * diffusion current) while the neighbours' edges are on their way */
+#pragma unroll
+        for (int k = 0; k < MK_CCPT; ++k) {
+            if (act[k]) {
+                const long long c = c0 + t + 64 * k;
+                if (p.row >= 0 && a.nan_check && c == a.nan_cell &&
+                    y[k][0] != y[k][0])
+                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
+                const Real pace = paced[k] ? p.pace : (Real)0;
+                /* logged values are register copies, written out only at
+                 * log steps */
+                mk_cell_a<true>(y[k], p.t, p.dt, pace, fv[k], q, cy[k]);
+                if (p.row >= 0) mk_write_log_part<0>(a, c, p.row, q);
+            }
+            /* one cell after the other: interleaving the cells' part A
+             * buys no latency the other waves do not hide, and multiplies
+             * the live registers */
+            __builtin_amdgcn_sched_barrier(0);
+        }
         if (i > 0 && (nl || nr)) {
             /* the connected neighbours' edge cells of the previous step
              * into the halo */
             const int step = ch.g0 + i;
-            const int par = (step - 1) & 1;
+            const int hpar = (step - 1) & 1;
             if ((t == 0 && nl) || (t == 1 && nr)) {
                 Real v;
-                if (mk_tget(t == 0 ? mk_cedge(ch, par, tile - 1, 1)
-                                   : mk_cedge(ch, par, tile + 1, 0),
+                if (mk_tget(t == 0 ? mk_cedge(ch, hpar, tile - 1, 1)
+                                   : mk_cedge(ch, hpar, tile + 1, 0),
                             step, ch.spin_limit, v)) {
                     vc[cur][t == 0 ? 0 : len + 1] = v;
                 } else {
@@ -726,9 +815,8 @@ mk_chain(const MkArgs a, const MkChain ch, const int s0, const int s1)
             __syncthreads();
             if (wg_flag) return;
         }
-        const MkStep p = a.steps[s];
-        const bool last = s + 1 == s1;
-        const int par = (ch.g0 + i) & 1;
+        /* Part B: the diffusion current from the potentials at t, then the
+         * rest of the step */
 #pragma unroll
         for (int k = 0; k < MK_CCPT; ++k) {
             const int li = t + 64 * k;
@@ -747,13 +835,8 @@ mk_chain(const MkArgs a, const MkChain ch, const int s0, const int s1)
                     if (cf[k] & 1u) id = id + gl[k] * (v - vl);
                 }
                 const Real pace = paced[k] ? p.pace : (Real)0;
-                if (p.row >= 0 && a.nan_check && c == a.nan_cell &&
-                    y[k][0] != y[k][0])
-                    __atomic_store_n(a.halt, p.row, __ATOMIC_RELAXED);
-                /* One instance of the cell code: logged values are register
-                 * copies, written out only at log steps */
-                mk_cell<true>(y[k], p.t, p.dt, pace, id, fv[k], q);
-                if (p.row >= 0) mk_write_log(a, c, p.row, q);
+                mk_cell_b<true>(y[k], p.t, p.dt, pace, id, fv[k], q, cy[k]);
+                if (p.row >= 0) mk_write_log_part<1>(a, c, p.row, q);
                 const Real vn = y[k][MK_IVM];
                 vc[cur ^ 1][li + 1] = vn;
                 if (!last) {

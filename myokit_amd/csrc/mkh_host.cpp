@@ -74,6 +74,10 @@ struct Program {
     hipFunction_t f_adj_step = nullptr, f_adj_block = nullptr,
                   f_adj_probe = nullptr, f_adj_sheet = nullptr,
                   f_adj_gather = nullptr;
+    /* mk_layout of the module: precision, states, fields, sheet / chain /
+     * reverse-sheet cells per thread (have_layout false if absent) */
+    int layout[6] = {0, 0, 0, 0, 0, 0};
+    bool have_layout = false;
     std::vector<char> code;
 };
 
@@ -1514,13 +1518,15 @@ struct Sim : SimBase {
                               driver == MKH_DRIVER_BLOCK ||
                               driver == MKH_DRIVER_CELLS;
         const bool block_rev = driver == MKH_DRIVER_BLOCK;
-        /* The persistent reverse sheet: 64 x 4 MK_ACPT tiles, all
-         * co-resident; they are no more than the forward sheet's (4 *
-         * sheet_cpt rows), whose edge buffer they use */
+        /* The persistent reverse sheet: 64 x 4 adj_cpt tiles (MK_ACPT in
+         * the kernel source), all co-resident; they are no more than the
+         * forward sheet's (4 * sheet_cpt rows), whose edge buffer they use */
         const int rtx = sh_tx;
-        const int rty = (int)((cfg.ny + 4 * MK_ACPT - 1) / (4 * MK_ACPT));
+        const int ath = 4 * std::max(cfg.adj_cpt, 1);
+        const int rty = (int)((cfg.ny + ath - 1) / ath);
         bool sheet_rev = false;
-        if (driver == MKH_DRIVER_SHEET && prog->f_adj_sheet &&
+        if (driver == MKH_DRIVER_SHEET && cfg.adj_cpt > 0 &&
+            prog->f_adj_sheet &&
             sh_edges.p && (size_t)rtx * rty * 4 * 64 * 2 * XW <= sh_edges.n &&
             (size_t)rtx * rty <= nparts) {
             int per_cu = 0;
@@ -2297,6 +2303,15 @@ int mkh_program_build(const char* source, const char* options,
             prog->f_adj_sheet = nullptr;
             (void)hipGetLastError();
         }
+        hipDeviceptr_t lp = nullptr;
+        size_t lsz = 0;
+        if (hipModuleGetGlobal(&lp, &lsz, prog->module, "mk_layout") ==
+                hipSuccess && lsz == sizeof(prog->layout)) {
+            MKH_HIP(hipMemcpyDtoH(prog->layout, lp, sizeof(prog->layout)));
+            prog->have_layout = true;
+        } else {
+            (void)hipGetLastError();
+        }
     }
     *program = prog.release();
     return 0;
@@ -2324,6 +2339,31 @@ int mkh_sim_create(void* program, const mkh_sim_config* cfg, void** sim)
         return fail(MKH_E_ARG, "block_threads must be a multiple of 64");
     if (cfg->precision != 32 && cfg->precision != 64)
         return fail(MKH_E_ARG, "precision must be 32 or 64");
+    if (p->have_layout) {
+        /* the configuration must describe the layout the kernel text was
+         * generated for (a zero cells-per-thread turns that driver off) */
+        const int* L = p->layout;
+        const char* what = nullptr;
+        int want = 0, got = 0;
+        if (cfg->precision != L[0]) {
+            what = "precision"; want = L[0]; got = cfg->precision;
+        } else if (cfg->n_state != L[1]) {
+            what = "n_state"; want = L[1]; got = cfg->n_state;
+        } else if (cfg->n_field != L[2]) {
+            what = "n_field"; want = L[2]; got = cfg->n_field;
+        } else if (cfg->sheet_cpt > 0 && cfg->sheet_cpt != L[3]) {
+            what = "sheet_cpt"; want = L[3]; got = cfg->sheet_cpt;
+        } else if (cfg->chain_cpt > 0 && cfg->chain_cpt != L[4]) {
+            what = "chain_cpt"; want = L[4]; got = cfg->chain_cpt;
+        } else if (cfg->adj_cpt > 0 && cfg->adj_cpt != L[5]) {
+            what = "adj_cpt"; want = L[5]; got = cfg->adj_cpt;
+        }
+        if (what)
+            return fail(MKH_E_ARG, std::string(what) + " " +
+                                       std::to_string(got) +
+                                       " does not match the kernel's " +
+                                       std::to_string(want));
+    }
     SimBase* s;
     int r;
     if (cfg->precision == 64) {

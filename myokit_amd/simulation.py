@@ -734,7 +734,7 @@ class SimulationOpenCL:
 
     def set_tuning(self, driver=None, block_threads=None, sheet_cpt=None,
                    sheet_waves=None, adjoint_waves=None, chain_cpt=None,
-                   chain_waves=None):
+                   chain_waves=None, adjoint_cpt=None):
         """Launch configuration of the device engine (chosen automatically
         by default; :func:`myokit_amd.tune.autotune` measures and picks
         one). ``driver``: ``'auto'``, ``'cells'``, ``'block'``, ``'grid'``,
@@ -746,7 +746,9 @@ class SimulationOpenCL:
         driver (graphs of disjoint chains; one-wave tiles of at most
         64*cpt cells); ``adjoint_waves``: the waves per SIMD of the reverse
         kernels of :class:`TissueAdjoint` (0: as many as their registers
-        allow). Arguments left as None keep their current value."""
+        allow); ``adjoint_cpt``: cells per thread of the persistent reverse
+        sheet (tile 64 x 4*cpt). Arguments left as None keep their current
+        value."""
         names = {'auto': -1, 'cells': _native.DRIVER_CELLS,
                  'block': _native.DRIVER_BLOCK, 'grid': _native.DRIVER_GRID,
                  'sheet': _native.DRIVER_SHEET,
@@ -775,6 +777,12 @@ class SimulationOpenCL:
             if not 1 <= int(chain_waves) <= 8:
                 raise ValueError('chain_waves must be in [1, 8].')
             self._tuning['chain_waves'] = int(chain_waves)
+        if adjoint_cpt is not None:
+            if int(adjoint_cpt) not in _kernelgen.ADJ_SHEET_CPT_CHOICES:
+                raise ValueError('adjoint_cpt must be one of '
+                                 + str(_kernelgen.ADJ_SHEET_CPT_CHOICES)
+                                 + '.')
+            self._tuning['adjoint_cpt'] = int(adjoint_cpt)
         if adjoint_waves is not None:
             if not 0 <= int(adjoint_waves) <= 8:
                 raise ValueError('adjoint_waves must be in [0, 8].')
@@ -843,7 +851,10 @@ class SimulationOpenCL:
                     'chain_cpt', _kernelgen.chain_cpt(self._precision)),
                 'chain_waves': self._tuning.get(
                     'chain_waves', _kernelgen.chain_waves(self._precision)),
-                'adjoint_waves': self._tuning.get('adjoint_waves')}
+                'adjoint_waves': self._tuning.get('adjoint_waves'),
+                'adjoint_cpt': self._tuning.get(
+                    'adjoint_cpt',
+                    _kernelgen.adjoint_sheet_cpt(self._precision))}
 
     def set_engine_option(self, key, value):
         """Tuning option passed to the device library (``batch_steps``,
@@ -908,7 +919,8 @@ class SimulationOpenCL:
             tile_waves=tune.get('sheet_waves'),
             adjoint_waves=tune.get('adjoint_waves'),
             chain_cells=tune.get('chain_cpt'),
-            chain_budget=tune.get('chain_waves'))
+            chain_budget=tune.get('chain_waves'),
+            adjoint_cpt=tune.get('adjoint_cpt'))
         if len(self._sources) >= 16:
             self._sources.pop(next(iter(self._sources)))
         self._sources[gkey] = (src, cm)
@@ -976,6 +988,8 @@ class SimulationOpenCL:
             else 0
         cfg.chain_cpt = tune.get('chain_cpt') or \
             _kernelgen.chain_cpt(self._precision)
+        cfg.adj_cpt = tune.get('adjoint_cpt') or \
+            _kernelgen.adjoint_sheet_cpt(self._precision)
         eng = _Engine(prog, cfg)
         eng.options = {}
         eng.config_version = None

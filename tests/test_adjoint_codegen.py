@@ -430,22 +430,32 @@ def test_legacy_multiply_is_compiler_visible(tmp_path):
     assert n >= 3 * 50      # three reverse kernels, dozens of guards each
 
 
-def test_reverse_sheet_tile_height_matches_header():
-    # The generator sizes the reverse sheet's LDS with ADJ_SHEET_CPT; the
-    # kernel and the host library take MK_ACPT from csrc/mk_args.h.
-    import re
-    text = open(os.path.join(CSRC, 'mk_args.h')).read()
-    m = re.search(r'#define MK_ACPT (\d+)', text)
-    assert m and int(m.group(1)) == _kernelgen.ADJ_SHEET_CPT
+def test_reverse_sheet_tile_height_and_lds_budget():
+    # The generator writes the reverse sheet's cells per thread as MK_ACPT
+    # and sizes its workgroups per CU from the tile's LDS: mu (8 states) of
+    # a 64 x 4 cpt tile and abar with a one-cell ring, in 160 KiB per CU
     m = myokit.load_model('example')
     s = myokit.SimulationOpenCL(m, None, ncells=(64, 32), precision=32,
                                 rl=True)
     cm = _kernelgen.CellModel(s._model, s._bound_variables, s._rl_states,
                               [], [s._model.get('membrane.V')])
-    assert _kernelgen.adjoint_sheet_groups(cm, 32) == 2
+    for cpt, groups in ((8, 2), (4, 4), (2, 4)):
+        lds = 8 * 4 * cpt * 64 * 4 + (4 * cpt + 2) * 66 * 4 + 68
+        assert groups == min(4, 160 * 1024 // lds)
+        assert _kernelgen.adjoint_sheet_groups(cm, 32, cpt) == groups
+        src = _kernelgen.generate_kernel(cm, 32, 1, adjoint=True,
+                                         adjoint_cpt=cpt)
+        assert '#define MK_ADJ_SHEET 1' in src
+        assert '#define MK_ACPT %d' % cpt in src
+        assert '#define MK_ADJ_SWAVES %d' % groups in src
     src = _kernelgen.generate_kernel(cm, 32, 1, adjoint=True)
-    assert '#define MK_ADJ_SHEET 1' in src
-    assert '#define MK_ADJ_SWAVES 2' in src
+    assert '#define MK_ACPT %d' % _kernelgen.adjoint_sheet_cpt(32) in src
+    with pytest.raises(ValueError, match='adjoint_cpt'):
+        _kernelgen.generate_kernel(cm, 32, 1, adjoint=True, adjoint_cpt=3)
+    with pytest.raises(ValueError, match='adjoint_cpt'):
+        s.set_tuning(adjoint_cpt=16)
+    s.set_tuning(adjoint_cpt=8)
+    assert s.tuning()['adjoint_cpt'] == 8
 
 
 def test_island_window_edges(tmp_path):

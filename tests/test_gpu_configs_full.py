@@ -248,26 +248,28 @@ def test_config2_s2_switch_against_oracle():
     assert np.max(outs[1][:5, 0]) > 0
 
 
-def test_config2_full_s1_s2_fp32_against_fp64():
-    """Configuration 2 as specified: 1024 x 1024, RL, dt 0.01 ms, S1 on the
-    left 4 columns at 0 ms, S2 on the lower-left quadrant at 320 ms (set
-    between two runs), 2000 ms, V of every cell every 2 ms. Single
-    precision against double precision on the device (the oracle would
-    need hours; the double-precision device path equals the oracle bit for
-    bit on the windows above). Reports V RMS over the whole run, the first
-    activation time of every cell, and the activations after S2."""
+def _config2_run(s2_time, k_max=8):
+    """Configuration 2 at 1024 x 1024 in both precisions on the device, S2
+    on the lower-left quadrant at ``s2_time`` (set between two runs), 2000
+    ms, V of every cell every 2 ms. Returns (V RMS of single against double
+    precision, activation times (k_max, n) per precision: the k-th upward
+    crossing of -30 mV of each cell, NaN where there is none)."""
     shape = (1024, 1024)
-    sims = {prec: _s1s2(shape, prec, 320.0)[0] for prec in (64, 32)}
+    sims = {prec: _s1s2(shape, prec, s2_time)[0] for prec in (64, 32)}
     n = shape[0] * shape[1]
     sq = 0.0
     count = 0
     prev = {}
-    first = {prec: np.full(n, np.nan) for prec in sims}
-    later = {prec: np.zeros(n, dtype=np.int64) for prec in sims}
+    acts = {prec: np.full((k_max, n), np.nan) for prec in sims}
+    seen = {prec: np.zeros(n, dtype=np.int64) for prec in sims}
     t_last = None
-    t0 = time.time()
-    for seg in [320.0] + [100.0] * 16 + [80.0]:
-        if abs(sims[64].time() - 320.0) < 1e-9:
+    segs, t = [], 0.0
+    for stop in sorted({s2_time, 2000.0} | set(np.arange(100.0, 2000.0,
+                                                         100.0))):
+        segs.append(stop - t)
+        t = stop
+    for seg in segs:
+        if abs(sims[64].time() - s2_time) < 1e-9:
             for s in sims.values():
                 s.set_paced_cells(512, 512, 0, 0)
         blocks = {}
@@ -292,32 +294,105 @@ def test_config2_full_s1_s2_fp32_against_fp64():
             v0, v1 = V[rows_, cells], V[rows_ + 1, cells]
             ta = tt[rows_] + (-30 - v0) / (v1 - v0) * (tt[rows_ + 1]
                                                        - tt[rows_])
-            # the first crossing of each cell in this block
             order = np.lexsort((ta, cells))
             cs, ts = cells[order], ta[order]
-            firsts = np.ones(len(cs), dtype=bool)
-            firsts[1:] = cs[1:] != cs[:-1]
-            fc, ft = cs[firsts], ts[firsts]
-            unset = np.isnan(first[prec][fc])
-            first[prec][fc[unset]] = ft[unset]
-            later[prec] += np.bincount(cells, minlength=n)
-            later[prec][fc[unset]] -= 1
+            start = np.ones(len(cs), dtype=bool)
+            start[1:] = cs[1:] != cs[:-1]
+            first_idx = np.maximum.accumulate(
+                np.where(start, np.arange(len(cs)), 0))
+            k = seen[prec][cs] + np.arange(len(cs)) - first_idx
+            keep = k < k_max
+            acts[prec][k[keep], cs[keep]] = ts[keep]
+            seen[prec] += np.bincount(cells, minlength=n)
             prev[prec] = blocks[prec][1][-1].astype(np.float64)
         t_last = t[-1]
-    el = time.time() - t0
-    rms = float(np.sqrt(sq / count))
-    a, b = first[32], first[64]
-    both = np.isfinite(a) & np.isfinite(b)
-    dact = float(np.max(np.abs(a[both] - b[both])))
-    print('config2 1024^2 S1+S2 2000 ms (%.0f s): fp32 vs fp64 V RMS %.3g mV;'
-          ' first activation max|dt| %.3g ms over %d cells; activations'
-          ' after the first: fp64 %d, fp32 %d (cells with any: %d, %d)'
-          % (el, rms, dact, int(both.sum()), int(later[64].sum()),
-             int(later[32].sum()), int(np.sum(later[64] > 0)),
-             int(np.sum(later[32] > 0))), flush=True)
-    assert np.array_equal(np.isfinite(a), np.isfinite(b))
-    assert dact <= 0.1
+    return float(np.sqrt(sq / count)), acts, seen
+
+
+def _compare_activations(acts, seen):
+    """Statistics of single against double precision activation times:
+    ``dact`` the largest |dt| over the activations of cells that activate
+    equally often in both, ``p999`` its 99.9th percentile, ``over`` how many
+    of them differ by more than 0.1 ms, ``differ`` the cells whose counts
+    differ, ``later`` the activations after the first (double precision)."""
+    a, b = acts[32], acts[64]
+    same = seen[32] == seen[64]
+    both = np.isfinite(a) & np.isfinite(b) & same[None, :]
+    d = np.abs(a[both] - b[both])
+    f = np.isfinite(a[0]) & np.isfinite(b[0])
+    return {'first': float(np.max(np.abs(a[0][f] - b[0][f]))),
+            'dact': float(d.max()) if d.size else 0.0,
+            'p999': float(np.percentile(d, 99.9)) if d.size else 0.0,
+            'over': int(np.sum(d > 0.1)), 'n': int(d.size),
+            'differ': int(np.sum(~same)),
+            'later': int(np.sum(np.maximum(seen[64] - 1, 0))),
+            'most': int(seen[64].max())}
+
+
+def test_config2_full_s1_s2_fp32_against_fp64():
+    """Configuration 2 as specified: 1024 x 1024, RL, dt 0.01 ms, S1 on the
+    left 4 columns at 0 ms, S2 on the lower-left quadrant at 320 ms (set
+    between two runs), 2000 ms, V of every cell every 2 ms. Single
+    precision against double precision on the device (the oracle would
+    need hours; the double-precision device path equals the oracle bit for
+    bit on the windows above). Reports V RMS over the whole run and the
+    activation times of every cell.
+
+    With the bundled LR1991 the S1 wave crosses 100 cells in 17 ms and a
+    cell repolarises below -70 mV about 390 ms after its upstroke, so at
+    320 ms the whole quadrant is still refractory: S2 excites nothing, and
+    every cell activates once (checked in both precisions)."""
+    t0 = time.time()
+    rms, acts, seen = _config2_run(320.0)
+    c = _compare_activations(acts, seen)
+    print('config2 1024^2 S1 + S2 at 320 ms, 2000 ms (%.0f s): fp32 vs fp64 '
+          'V RMS %.3g mV; activation max|dt| %.3g ms over %d activations; '
+          'cells with different activation counts %d; activations after the '
+          'first (fp64) %d' % (time.time() - t0, rms, c['dact'], c['n'],
+                               c['differ'], c['later']), flush=True)
+    assert c['differ'] == 0 and c['later'] == 0
+    assert np.all(seen[64] == 1)
+    assert c['dact'] <= 0.1
     assert rms <= 0.5
+
+
+def test_config2_reentry_fp32_against_fp64():
+    """The same sheet with S2 at 440 ms, when the S1 wave's back (V below
+    -70 mV) has passed about a third of the quadrant: S2 excites the
+    recovered part of the quadrant, which propagates into the recovered
+    tissue above it while the still refractory part to its right blocks
+    it, and re-entry follows (about a million activations after the
+    first). The regime where single precision is hardest to keep beside
+    double precision; every activation of every cell is compared.
+
+    Measured on MI355X: V RMS 0.137 mV over the 2000 ms; the first
+    activation of every cell (the S1 wave) within the 0.1 ms bar; of the
+    2.1 million activations of cells that activate equally often in both
+    precisions, 6.4% differ by more than 0.1 ms (99.9th percentile 0.57
+    ms, largest 2.2 ms): the re-entrant wave arrives in tissue that is
+    still recovering, where a small difference in the gates' recovery moves
+    the upstroke by much more; 145 cells of 1,048,576, at the edge of the
+    conduction block, activate once more or less in one precision. The
+    bars below hold the single-precision bars where they apply (RMS, the
+    first activation) and guard the rest against regressions."""
+    t0 = time.time()
+    rms, acts, seen = _config2_run(440.0)
+    c = _compare_activations(acts, seen)
+    n = seen[64].size
+    print('config2 1024^2 S1 + S2 at 440 ms, 2000 ms (%.0f s): fp32 vs fp64 '
+          'V RMS %.3g mV; first activation max|dt| %.3g ms; over the %d '
+          'activations of cells that activate equally often: max|dt| %.3g '
+          'ms, 99.9th percentile %.3g ms, %d beyond 0.1 ms; cells with '
+          'different activation counts %d of %d; activations after the first'
+          ' (fp64) %d; most activations of one cell %d'
+          % (time.time() - t0, rms, c['first'], c['n'], c['dact'], c['p999'],
+             c['over'], c['differ'], n, c['later'], c['most']), flush=True)
+    assert c['later'] > 0
+    assert rms <= 0.5
+    assert c['first'] <= 0.1
+    assert c['differ'] <= 1e-3 * n
+    assert c['p999'] <= 1.0 and c['dact'] <= 5.0
+    assert c['over'] <= 0.1 * c['n']
 
 
 @pytest.mark.parametrize('driver', ['block', 'grid'])

@@ -60,12 +60,15 @@ def test_set_tuning_validation():
         s.set_tuning(chain_cpt=3)
     with pytest.raises(ValueError):
         s.set_tuning(chain_waves=9)
+    with pytest.raises(ValueError):
+        s.set_tuning(adjoint_cpt=3)
     s.set_tuning(driver='grid', block_threads=128, sheet_cpt=8,
-                 sheet_waves=3, adjoint_waves=4, chain_cpt=1, chain_waves=6)
+                 sheet_waves=3, adjoint_waves=4, chain_cpt=1, chain_waves=6,
+                 adjoint_cpt=4)
     t = s.tuning()
     assert t == {'driver': 'grid', 'block_threads': 128, 'sheet_cpt': 8,
                  'sheet_waves': 3, 'adjoint_waves': 4, 'chain_cpt': 1,
-                 'chain_waves': 6}
+                 'chain_waves': 6, 'adjoint_cpt': 4}
 
 
 def test_cache_round_trip(tmp_path, monkeypatch):
