@@ -29,24 +29,25 @@ CLOCK_GHZ = 2.4              # MI355X peak engine clock
 SIMDS = 256 * 4              # 256 CUs x 4 SIMDs
 
 # mk_sheet on this workload (FP32), from rocprofv3 counters of 200-step
-# dispatches taken at commit 4c02761
-# (profiles/mk_sheet_lr1991_1024x1024_fp32.md):
+# dispatches taken at commit 1e38df8
+# (profiles/mk_sheet_lr1991_1024x1024_fp32.md), medians of 26 dispatches:
 # - HBM traffic: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors of a 256 MiB
-#   device copy, bench.py --copy-calibration), 182,520 x 2 + 444,420 KiB per
+#   device copy, bench.py --copy-calibration), 163,300 x 2 + 447,500 KiB per
 #   200 x 2^20 cell-steps;
-# - per wave-step of 4 cells: 1,105 VALU instructions, 224 of them
-#   transcendental; a transcendental holds the SIMD 8 cycles, any other VALU
-#   instruction 4 (SQ_ACTIVE_INST_VALU = VALU - TRANS + 2 TRANS);
-# - VALU busy (SQ_ACTIVE_INST_VALU x 4 / SIMDs / cycles per XCD): 0.77.
+# - SQ_INSTS_VALU 9.507e8 and SQ_INSTS_VALU_TRANS_F32 1.835e8 over 4,096
+#   waves x 200 steps (wave-steps of 4 cells); a transcendental holds the
+#   SIMD 8 cycles, any other VALU instruction 4 (SQ_ACTIVE_INST_VALU =
+#   VALU - TRANS + 2 TRANS);
+# - VALU busy (SQ_ACTIVE_INST_VALU x 4 / SIMDs / cycles per XCD): 0.87.
 SHEET_PROFILE = {'file': 'profiles/mk_sheet_lr1991_1024x1024_fp32.md',
-                 'commit': '4c02761'}
-SHEET_TRAFFIC_B_PER_CELL_STEP = ((182520 * 2 + 444420) * 1024.0
+                 'commit': '1e38df8'}
+SHEET_TRAFFIC_B_PER_CELL_STEP = ((163300 * 2 + 447500) * 1024.0
                                  / (200 * 2.0**20))
-SHEET_VALU = 1105
-SHEET_TRANS = 224
+SHEET_VALU = 9.507e8 / (4096 * 200)
+SHEET_TRANS = 1.835e8 / (4096 * 200)
 SHEET_VALU_CYCLES_PER_WAVE_STEP = (SHEET_VALU - SHEET_TRANS) * 4 \
     + SHEET_TRANS * 8
-SHEET_VALU_BUSY = 0.77
+SHEET_VALU_BUSY = 0.87
 SHEET_CELLS_PER_LANE = 4
 
 
@@ -603,9 +604,9 @@ def main():
             'measured_us_per_time_step': meas_us,
             'frac': model_us / meas_us,
             'busy_counter': SHEET_VALU_BUSY,
-            'model': '(%d - %d) VALU x 4 + %d transcendental x 8 cycles per '
-                     'wave-step of 4 cells, 4 SIMDs x 256 CUs, 2.4 GHz' % (
-                         SHEET_VALU, SHEET_TRANS, SHEET_TRANS),
+            'model': '(%.0f - %.0f) VALU x 4 + %.0f transcendental x 8 '
+                     'cycles per wave-step of 4 cells, 4 SIMDs x 256 CUs, '
+                     '2.4 GHz' % (SHEET_VALU, SHEET_TRANS, SHEET_TRANS),
         }
         line['roofline']['profile'] = SHEET_PROFILE
     if args.adjoint_ms > 0:

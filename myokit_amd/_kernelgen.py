@@ -852,6 +852,12 @@ def sheet_waves(precision):
     return 2 if precision == DOUBLE_PRECISION else 4
 
 
+def grid_waves(precision):
+    """Waves per SIMD the grid driver's register budget must allow (0: no
+    limit, the compiler's choice)."""
+    return 0
+
+
 def sheet_enabled(precision):
     return True
 
@@ -884,7 +890,7 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
                     native_maths=False, adjoint=False, tile_cpt=None,
                     tile_waves=None, adjoint_precision=None,
                     adjoint_waves=None, chain_cells=None, chain_budget=None,
-                    adjoint_cpt=None):
+                    adjoint_cpt=None, grid_budget=None):
     """Returns the complete HIP source for hiprtc. ``tile_cpt`` (cells
     per thread of the persistent sheet: tile height 4 x cpt) and
     ``tile_waves`` (waves per SIMD its register budget allows) default to
@@ -896,7 +902,9 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
     ``chain_cells`` and ``chain_budget`` set the chain driver's cells per
     thread and waves per SIMD (defaults :func:`chain_cpt`,
     :func:`chain_waves`), ``adjoint_cpt`` the persistent reverse sheet's
-    cells per thread (default :func:`adjoint_sheet_cpt`)."""
+    cells per thread (default :func:`adjoint_sheet_cpt`), ``grid_budget``
+    the waves per SIMD of the grid driver's register budget (default
+    :func:`grid_waves`; 0: no limit)."""
     if block_threads % 64 or not 64 <= block_threads <= 1024:
         raise ValueError('block_threads must be a multiple of 64 in'
                          ' [64, 1024].')
@@ -926,6 +934,11 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         raise ValueError('adjoint_cpt must be one of '
                          + str(ADJ_SHEET_CPT_CHOICES) + '.')
     asg = adjoint_sheet_groups(cm, precision, acpt)
+    gw = grid_waves(precision) if grid_budget is None else int(grid_budget)
+    if not 0 <= gw <= 8:
+        raise ValueError('grid_waves must be in [0, 8].')
+    # workgroups per CU for that many waves per SIMD (4 SIMDs per CU)
+    gblocks = max(1, (gw * 256) // int(block_threads)) if gw else 0
     parts = [
         '// Generated by myokit_amd for model: '
         + str(cm.model.name()).replace('\n', ' '),
@@ -941,6 +954,7 @@ def generate_kernel(cm, precision, coupling, block_threads=256,
         '#define MK_CCPT ' + str(ccpt),
         '#define MK_CWAVES ' + str(cwaves),
         '#define MK_ACPT ' + str(acpt),
+        '#define MK_GBLOCKS ' + str(gblocks),
         '#define MK_ADJ_SHEET ' + ('1' if adjoint and int(coupling) == 1
                                    and asg > 0
                                    and adjoint_precision in (None, precision)

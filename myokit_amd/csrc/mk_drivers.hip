@@ -303,7 +303,15 @@ __device__ __forceinline__ long long mk_xcd_block(long long b, long long nb)
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-extern "C" __global__ void __launch_bounds__(MK_BT)
+#ifndef MK_GBLOCKS
+#define MK_GBLOCKS 0
+#endif
+#if MK_GBLOCKS > 0
+#define MK_GRID_BOUNDS __launch_bounds__(MK_BT, MK_GBLOCKS)
+#else
+#define MK_GRID_BOUNDS __launch_bounds__(MK_BT)
+#endif
+extern "C" __global__ void MK_GRID_BOUNDS
 mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
         const int s)
 {

@@ -734,7 +734,7 @@ class SimulationOpenCL:
 
     def set_tuning(self, driver=None, block_threads=None, sheet_cpt=None,
                    sheet_waves=None, adjoint_waves=None, chain_cpt=None,
-                   chain_waves=None, adjoint_cpt=None):
+                   chain_waves=None, adjoint_cpt=None, grid_waves=None):
         """Launch configuration of the device engine (chosen automatically
         by default; :func:`myokit_amd.tune.autotune` measures and picks
         one). ``driver``: ``'auto'``, ``'cells'``, ``'block'``, ``'grid'``,
@@ -747,8 +747,9 @@ class SimulationOpenCL:
         64*cpt cells); ``adjoint_waves``: the waves per SIMD of the reverse
         kernels of :class:`TissueAdjoint` (0: as many as their registers
         allow); ``adjoint_cpt``: cells per thread of the persistent reverse
-        sheet (tile 64 x 4*cpt). Arguments left as None keep their current
-        value."""
+        sheet (tile 64 x 4*cpt); ``grid_waves``: the waves per SIMD the grid
+        driver's register budget allows (0: no limit). Arguments left as
+        None keep their current value."""
         names = {'auto': -1, 'cells': _native.DRIVER_CELLS,
                  'block': _native.DRIVER_BLOCK, 'grid': _native.DRIVER_GRID,
                  'sheet': _native.DRIVER_SHEET,
@@ -777,6 +778,10 @@ class SimulationOpenCL:
             if not 1 <= int(chain_waves) <= 8:
                 raise ValueError('chain_waves must be in [1, 8].')
             self._tuning['chain_waves'] = int(chain_waves)
+        if grid_waves is not None:
+            if not 0 <= int(grid_waves) <= 8:
+                raise ValueError('grid_waves must be in [0, 8].')
+            self._tuning['grid_waves'] = int(grid_waves)
         if adjoint_cpt is not None:
             if int(adjoint_cpt) not in _kernelgen.ADJ_SHEET_CPT_CHOICES:
                 raise ValueError('adjoint_cpt must be one of '
@@ -854,7 +859,9 @@ class SimulationOpenCL:
                 'adjoint_waves': self._tuning.get('adjoint_waves'),
                 'adjoint_cpt': self._tuning.get(
                     'adjoint_cpt',
-                    _kernelgen.adjoint_sheet_cpt(self._precision))}
+                    _kernelgen.adjoint_sheet_cpt(self._precision)),
+                'grid_waves': self._tuning.get(
+                    'grid_waves', _kernelgen.grid_waves(self._precision))}
 
     def set_engine_option(self, key, value):
         """Tuning option passed to the device library (``batch_steps``,
@@ -920,7 +927,8 @@ class SimulationOpenCL:
             adjoint_waves=tune.get('adjoint_waves'),
             chain_cells=tune.get('chain_cpt'),
             chain_budget=tune.get('chain_waves'),
-            adjoint_cpt=tune.get('adjoint_cpt'))
+            adjoint_cpt=tune.get('adjoint_cpt'),
+            grid_budget=tune.get('grid_waves'))
         if len(self._sources) >= 16:
             self._sources.pop(next(iter(self._sources)))
         self._sources[gkey] = (src, cm)

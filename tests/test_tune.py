@@ -62,13 +62,15 @@ def test_set_tuning_validation():
         s.set_tuning(chain_waves=9)
     with pytest.raises(ValueError):
         s.set_tuning(adjoint_cpt=3)
+    with pytest.raises(ValueError):
+        s.set_tuning(grid_waves=9)
     s.set_tuning(driver='grid', block_threads=128, sheet_cpt=8,
                  sheet_waves=3, adjoint_waves=4, chain_cpt=1, chain_waves=6,
-                 adjoint_cpt=4)
+                 adjoint_cpt=4, grid_waves=5)
     t = s.tuning()
     assert t == {'driver': 'grid', 'block_threads': 128, 'sheet_cpt': 8,
                  'sheet_waves': 3, 'adjoint_waves': 4, 'chain_cpt': 1,
-                 'chain_waves': 6, 'adjoint_cpt': 4}
+                 'chain_waves': 6, 'adjoint_cpt': 4, 'grid_waves': 5}
 
 
 def test_cache_round_trip(tmp_path, monkeypatch):
@@ -94,3 +96,20 @@ def test_tuned_kernel_source_has_tile_shape():
     assert '#define MK_SCPT 8' in src and '#define MK_SWAVES 3' in src
     with pytest.raises(ValueError):
         _kernelgen.generate_kernel(cm, 32, 1, tile_cpt=16)
+
+
+def test_grid_register_budget_in_kernel_source():
+    # waves per SIMD -> workgroups per CU of the grid driver's launch
+    # bounds (4 SIMDs per CU): 5 waves of 256-thread workgroups is 5 per
+    # CU, of 1024-thread workgroups 1; 0 leaves the bound to the compiler
+    from myokit_amd import _kernelgen
+    from myokit_amd.simulation import _prepare_cell_model
+    cm = _prepare_cell_model(myokit.load_model('example'), rl=True,
+                             diffusion=True, fields=[], quantities=[])
+    for threads, waves, blocks in ((256, 5, 5), (128, 4, 8), (1024, 5, 1),
+                                   (256, 0, 0)):
+        src = _kernelgen.generate_kernel(cm, 64, 1, threads,
+                                         grid_budget=waves)
+        assert '#define MK_GBLOCKS %d' % blocks in src
+    with pytest.raises(ValueError):
+        _kernelgen.generate_kernel(cm, 64, 1, grid_budget=9)
