@@ -854,8 +854,12 @@ def sheet_waves(precision):
 
 def grid_waves(precision):
     """Waves per SIMD the grid driver's register budget must allow (0: no
-    limit, the compiler's choice)."""
-    return 0
+    limit, the compiler's choice). Double precision: 5, where the LR1991
+    step takes 98 registers unbounded (4 waves) and 96 with two spilled;
+    on the 1024^2 sheet 1.04e10 cell-steps/s unbounded, 1.11e10 at 5 and
+    1.08e10 at 6 (24 spilled), bit-identical. Single precision needs no
+    bound (38 registers, 8 waves)."""
+    return 5 if precision == DOUBLE_PRECISION else 0
 
 
 def sheet_enabled(precision):

@@ -13,8 +13,9 @@ Candidates: the drivers that can run the geometry (uncoupled cells; one
 workgroup per domain; one step per launch; the persistent sheet; the
 persistent chains) crossed with threads per workgroup (128, 256, 512, 1024)
 for the first three, cells per thread (tile heights 8, 16, 32) and register
-budgets (waves per SIMD) for the sheet, and cells per thread and register
-budgets for the chains. A candidate that fails to build or launch is
+budgets (waves per SIMD) for the sheet, cells per thread and register
+budgets for the chains, and in double precision register budgets for the
+one-step-per-launch driver. A candidate that fails to build or launch is
 skipped.
 """
 import hashlib
@@ -83,6 +84,12 @@ def candidates(sim):
                                 'chain_waves': w})
         elif d == 'block' and n <= 64:
             out.append({'driver': 'block', 'block_threads': 64})
+        elif d == 'grid':
+            gws = (0, 4, 5, 6) if sim._precision == 64 else (0,)
+            for bt in bts:
+                for gw in gws:
+                    out.append({'driver': d, 'block_threads': bt,
+                                'grid_waves': gw})
         else:
             for bt in bts:
                 out.append({'driver': d, 'block_threads': bt})

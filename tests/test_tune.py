@@ -36,6 +36,11 @@ def test_candidates_by_coupling():
     for x in tune.candidates(sim((64, 64), precision=64)):
         if x['driver'] == 'sheet':
             assert x['sheet_cpt'] in (2, 4)
+    g64 = {x['grid_waves'] for x in tune.candidates(
+        sim((64, 64), precision=64)) if x['driver'] == 'grid'}
+    assert g64 == {0, 4, 5, 6}
+    assert {x['grid_waves'] for x in tune.candidates(sim((64, 64)))
+            if x['driver'] == 'grid'} == {0}
 
 
 def myokit_amd_choices():
