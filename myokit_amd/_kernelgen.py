@@ -707,6 +707,9 @@ def generate_split_cell_function(cm, precision, native_maths=False):
         (qb if in_b else qa).append(line)
         qpart.append(1 if in_b else 0)
     ua, ub = [], []
+    # a state that part B advances still holds its value at t in y when
+    # part B starts (part A does not write it): B reads it there
+    in_y = {}
     for i, s in enumerate(cm.states):
         if s in cm.rl_states:
             inf, tau = cm.rl_states[s]
@@ -714,10 +717,13 @@ def generate_split_cell_function(cm, precision, native_maths=False):
         else:
             in_b = s in dstates
         (ub if in_b else ua).append(_state_update(cm, w, precision, i, s))
+        if in_b:
+            in_y[_cname(s, 'x_')] = i
     # what part B reads of part A's values
     b_text = '\n'.join(b_lines + qb + ub)
     used = set(re.findall(r'[A-Za-z_][A-Za-z0-9_]*', b_text))
-    carry = [n for n in declared if n in used]
+    from_y = [n for n in declared if n in used and n in in_y]
+    carry = [n for n in declared if n in used and n not in in_y]
     out = ['#define MK_NCARRY ' + str(max(1, len(carry))),
            '__device__ constexpr int mk_qpart[' + str(max(1, len(qpart)))
            + '] = {' + ', '.join(str(x) for x in (qpart or [0])) + '};']
@@ -747,6 +753,8 @@ def generate_split_cell_function(cm, precision, native_maths=False):
     out.append('{')
     out.append(tab + '(void)t; (void)dt; (void)pace; (void)idiff; (void)fv;'
                ' (void)q; (void)carry;')
+    out += [tab + 'const Real ' + n + ' = y[' + str(in_y[n]) + '];'
+            for n in from_y]
     out += [tab + 'const Real ' + n + ' = carry[' + str(j) + '];'
             for j, n in enumerate(carry)]
     out += [tab + x for x in b_lines]

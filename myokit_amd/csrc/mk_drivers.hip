@@ -650,17 +650,26 @@ mk_sheet(const MkArgs a, const MkSheet sh, const int s0, const int s1)
                 mk_cell_b<true>(y[k], p.t, p.dt, pace, id, fv[k], q, cy);
                 if (p.row >= 0) mk_write_log_part<1>(a, c, p.row, q);
                 if (sh.traj) mk_store_traj(a, sh.traj, i, c, y[k]);
-                const Real vn = y[k][MK_IVM];
-                vt[cur ^ 1][r + 1][lx + 1] = vn;
-                if (!last) {
-                    const int step = sh.g0 + i + 1;
-                    if (r == 0) mk_tput(mk_edge(sh, par, tile, 0, lx), vn, step);
-                    if (r == MK_STH - 1)
-                        mk_tput(mk_edge(sh, par, tile, 1, lx), vn, step);
-                    if (lx == 0) mk_tput(mk_edge(sh, par, tile, 2, r), vn, step);
-                    if (lx == 63)
-                        mk_tput(mk_edge(sh, par, tile, 3, r), vn, step);
-                }
+                vt[cur ^ 1][r + 1][lx + 1] = y[k][MK_IVM];
+            }
+        }
+        /* The tile's edges of this step for its neighbours: the bottom row
+         * (wave 0's first cells), the top row (wave 3's last), and the left
+         * and right columns (lanes 0 and 63 of every wave, their cells one
+         * after the other) */
+        if (!last) {
+            const int step = sh.g0 + i + 1;
+            if (w == 0 && act[0])
+                mk_tput(mk_edge(sh, par, tile, 0, lx), y[0][MK_IVM], step);
+            if (w == 3 && act[MK_SCPT - 1])
+                mk_tput(mk_edge(sh, par, tile, 1, lx),
+                        y[MK_SCPT - 1][MK_IVM], step);
+            if (lx == 0 || lx == 63) {
+                MkXWord* ep = mk_edge(sh, par, tile, lx == 0 ? 2 : 3,
+                                      w * MK_SCPT);
+#pragma unroll
+                for (int k = 0; k < MK_SCPT; ++k)
+                    if (act[k]) mk_tput(ep + k * MK_XW, y[k][MK_IVM], step);
             }
         }
         /* The next step's barrier (after its halo) orders this step's LDS

@@ -136,7 +136,7 @@ def test_split_equals_whole_lr1991(tmp_path, prec, rl):
     # i_diff is the only logged quantity part B sets
     assert [lib.qpart(k) for k in range(len(names))] == [0, 1, 0, 0, 0]
     src = _kernelgen.generate_split_cell_function(cm, prec)
-    assert '#define MK_NCARRY 3' in src      # V, i_ion, i_stim
+    assert '#define MK_NCARRY 2' in src      # i_ion, i_stim (V is in y)
     y0 = [s.initial_value(as_float=True) for s in cm.states]
     _compare(lib, rt, dtype, y0, 1, len(names), 7)
 
