@@ -106,6 +106,49 @@ def _program(source, options, device=0):
     return p
 
 
+def _graph_copies(i, j, n, block_cells, min_copies=4):
+    """The number K of identical copies a graph of ``n`` cells with
+    connections ``(i, j)`` is made of: the largest K >= ``min_copies``
+    (dividing ``n``) such that no connection crosses a block of n / K
+    cells and every block holds the same connections as the first, shifted
+    by its offset (in any order); 1 if there is none, or if a copy has at
+    most ``block_cells`` cells or is a set of chains (the block and chain
+    drivers, which need the copies contiguous, then apply)."""
+    if len(i) == 0:
+        return 1
+    lo, hi = np.minimum(i, j), np.maximum(i, j)
+    divisors = set()
+    d = 1
+    while d * d <= n:
+        if n % d == 0:
+            divisors.update((d, n // d))
+        d += 1
+    for k in sorted(divisors, reverse=True):
+        if k < min_copies:
+            break
+        m = n // k
+        if m <= block_cells:
+            continue
+        b = lo // m
+        if not np.array_equal(b, hi // m):
+            continue
+        counts = np.bincount(b, minlength=k)
+        if not np.all(counts == counts[0]):
+            continue
+        key = (lo - b * m) * m + (hi - b * m)
+        order = np.lexsort((key, b))
+        keys = key[order].reshape(k, -1)
+        if not np.all(keys == keys[0]):
+            continue
+        first = b == 0
+        deg = np.bincount(np.concatenate([lo[first], hi[first]]),
+                          minlength=m)
+        if deg.max() <= 2:
+            return 1
+        return k
+    return 1
+
+
 class _Engine:
     """One device simulation handle for one compiled kernel."""
 
@@ -163,6 +206,11 @@ class SimulationOpenCL:
         self._state_arr = st
         self._dev_newer = False
 
+    def _max_block_cells(self):
+        """Cells of the largest domain the one-workgroup-per-domain driver
+        holds in LDS (MK_BCELLS)."""
+        return 2048 if self._precision == DOUBLE_PRECISION else 4096
+
     def _cell_order(self):
         """``(perm, iperm)``: the order in which the device holds the cells
         of a set_connections graph (device cell k is the user's cell
@@ -173,7 +221,16 @@ class SimulationOpenCL:
         then touch few cache lines (a random geometric graph of 262,144
         cells numbered at random has neighbours 87,000 indices apart on
         average, 330 after reordering). The per-cell arithmetic, the order
-        of each cell's connections included, is the same in every order."""
+        of each cell's connections included, is the same in every order.
+
+        A graph that is the disjoint union of identical copies (a batch of
+        parameter sets run as one simulation, each set a copy of the same
+        graph in its own block of cells) is held with the copies
+        interleaved, cell p of every copy side by side, each copy in the
+        reverse Cuthill-McKee order of the first: a wave then reads each
+        neighbour's potential for many copies in one contiguous segment.
+        Copies small enough for one workgroup each keep the block order,
+        which the one-workgroup-per-domain driver needs."""
         if self._connections is None or len(self._connections) == 0:
             return None, None
         c = self._order_cache
@@ -185,11 +242,19 @@ class SimulationOpenCL:
         n = self._ntotal
         i = a[:, 0].astype(np.int64)
         j = a[:, 1].astype(np.int64)
-        g = coo_matrix((np.ones(2 * len(a), dtype=np.int8),
+        copies = _graph_copies(i, j, n, self._max_block_cells())
+        m = n // copies
+        if copies > 1:
+            # one copy's order, the copies interleaved: device cell
+            # p * copies + k is cell p of copy k
+            i, j = i[i < m], j[i < m]
+        g = coo_matrix((np.ones(2 * len(i), dtype=np.int8),
                         (np.concatenate([i, j]), np.concatenate([j, i]))),
-                       shape=(n, n)).tocsr()
+                       shape=(m, m)).tocsr()
         perm = np.asarray(reverse_cuthill_mckee(g, symmetric_mode=True),
                           dtype=np.int64)
+        if copies > 1:
+            perm = (perm[:, None] + m * np.arange(copies)[None, :]).ravel()
         iperm = np.empty(n, dtype=np.int64)
         iperm[perm] = np.arange(n)
         if np.array_equal(perm, np.arange(n)):

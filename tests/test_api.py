@@ -294,6 +294,46 @@ def test_graph_cell_order(model):
     assert s._cell_order() == (None, None)
 
 
+def test_graph_copies_interleaved(model):
+    # A batch of identical graph copies (cells k m .. (k + 1) m - 1 for
+    # copy k) is held with the copies interleaved: device cell p K + k is
+    # copy k's cell at position p of the first copy's reverse Cuthill-McKee
+    # order. Copies of at most a workgroup's worth of cells, sets of chains
+    # and blocks that differ keep the plain order.
+    from myokit_amd.simulation import _graph_copies
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(8)
+    m, K = 5000, 6
+    pts = rng.uniform(0, 1, (m, 2))
+    pairs = cKDTree(pts).query_pairs(np.sqrt(6 / (np.pi * m)),
+                                     output_type='ndarray')
+    rows = np.concatenate([
+        np.concatenate([pairs + k * m,
+                        rng.uniform(5, 12, (len(pairs), 1))], axis=1)
+        for k in range(K)])
+    i, j = rows[:, 0].astype(np.int64), rows[:, 1].astype(np.int64)
+    assert _graph_copies(i, j, m * K, 4096) == K
+    # the copies' connections in another order still count
+    sh = rng.permutation(len(i))
+    assert _graph_copies(i[sh], j[sh], m * K, 4096) == K
+    assert _graph_copies(i, j, m * K, 8192) == 1        # one workgroup each
+    assert _graph_copies(i[:-1], j[:-1], m * K, 4096) == 1  # a copy differs
+    chain = np.arange(m * K - 1)
+    keep = (chain + 1) % m != 0
+    assert _graph_copies(chain[keep], chain[keep] + 1, m * K, 4096) == 1
+    s = myokit.SimulationOpenCL(model, ncells=m * K)
+    s.set_connections(rows)
+    perm, iperm = s._cell_order()
+    assert sorted(perm) == list(range(m * K))
+    np.testing.assert_array_equal(iperm[perm], np.arange(m * K))
+    # cell p of every copy side by side, the same position in each copy
+    P = perm.reshape(-1, K)
+    assert np.all(P - P[:, :1] == np.arange(K)[None, :] * m)
+    # and neighbours close together within a copy
+    d = np.abs(iperm[i] - iperm[j]) // K
+    assert np.mean(d) < 0.5 * np.mean(np.abs(i - j))
+
+
 def test_adjoint_probe_checks(model):
     # The loss keeps one probe per cell: a cell listed twice is an error
     # rather than a silently dropped loss term (the library rejects it too)
