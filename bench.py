@@ -29,21 +29,21 @@ CLOCK_GHZ = 2.4              # MI355X peak engine clock
 SIMDS = 256 * 4              # 256 CUs x 4 SIMDs
 
 # mk_sheet on this workload (FP32), from rocprofv3 counters of 200-step
-# dispatches taken at commit 1e38df8
+# dispatches taken at commit 37cfa24
 # (profiles/mk_sheet_lr1991_1024x1024_fp32.md), medians of 26 dispatches:
 # - HBM traffic: FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors of a 256 MiB
-#   device copy, bench.py --copy-calibration), 163,300 x 2 + 447,500 KiB per
+#   device copy, bench.py --copy-calibration), 153,900 x 2 + 446,500 KiB per
 #   200 x 2^20 cell-steps;
-# - SQ_INSTS_VALU 9.507e8 and SQ_INSTS_VALU_TRANS_F32 1.835e8 over 4,096
+# - SQ_INSTS_VALU 9.281e8 and SQ_INSTS_VALU_TRANS_F32 1.835e8 over 4,096
 #   waves x 200 steps (wave-steps of 4 cells); a transcendental holds the
 #   SIMD 8 cycles, any other VALU instruction 4 (SQ_ACTIVE_INST_VALU =
 #   VALU - TRANS + 2 TRANS);
 # - VALU busy (SQ_ACTIVE_INST_VALU x 4 / SIMDs / cycles per XCD): 0.87.
 SHEET_PROFILE = {'file': 'profiles/mk_sheet_lr1991_1024x1024_fp32.md',
-                 'commit': '1e38df8'}
-SHEET_TRAFFIC_B_PER_CELL_STEP = ((163300 * 2 + 447500) * 1024.0
+                 'commit': '37cfa24'}
+SHEET_TRAFFIC_B_PER_CELL_STEP = ((153900 * 2 + 446500) * 1024.0
                                  / (200 * 2.0**20))
-SHEET_VALU = 9.507e8 / (4096 * 200)
+SHEET_VALU = 9.281e8 / (4096 * 200)
 SHEET_TRANS = 1.835e8 / (4096 * 200)
 SHEET_VALU_CYCLES_PER_WAVE_STEP = (SHEET_VALU - SHEET_TRANS) * 4 \
     + SHEET_TRANS * 8

@@ -552,9 +552,12 @@ def exp_families(cm, w, max_power=24, v_range=200.0):
             plines.append('const Real ' + have[n] + ' = ' + have[pair[0]]
                           + ' * ' + have[pair[1]] + ';')
         # Issue cost: a member evaluated directly is one fma and one
-        # quarter-rate v_exp_f32 (4 + 16 cycles); in the family it is a
-        # share of the powers (4 each) and, if m != 0 and no literal factor
-        # absorbs exp(m), one multiply (4). Keep the family if it gains.
+        # v_exp_f32, priced 4 + 16 cycles; in the family it is a share of
+        # the powers (4 each) and, if m != 0 and no literal factor absorbs
+        # exp(m), one multiply (4). Keep the family if it gains. (The SQ
+        # counters show transcendentals at half rate on gfx950, 8 cycles;
+        # at that price LR1991's two-member family exp(0.05 V) only breaks
+        # even, and every other family still gains.)
         members = [(c, k, m) for c, (k, m) in found.items() if k in mem]
         saved = 20 * (len(members) - 1)
         added = 4 * len(plines) + 4 * sum(
