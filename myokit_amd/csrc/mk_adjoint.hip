@@ -89,6 +89,13 @@ __device__ __forceinline__ Real mk_adj_couple(
     }
 #elif MK_COUPLING == 3
     const Real ac = ab[self], v = V1[self];
+    /* rows of one copy for a graph of interleaved copies (see mk_graph) */
+    const int K = a.gcopies;
+    int lane = 0;
+    if (K > 1) {
+        row = self / K;
+        lane = self - row * K;
+    }
     int e = a.csr_ptr[row];
     const int e1 = a.csr_ptr[row + 1];
     /* four connections per round, loads first, sums in list order (as in
@@ -99,7 +106,7 @@ __device__ __forceinline__ Real mk_adj_couple(
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int t = a.csr_idx[e + k];
-            nb[k] = t < 0 ? ~t : t;
+            nb[k] = (t < 0 ? ~t : t) * K + lane;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -117,6 +124,7 @@ __device__ __forceinline__ Real mk_adj_couple(
     for (; e < e1; ++e) {
         int nb = a.csr_idx[e];
         if (nb < 0) nb = ~nb;
+        nb = nb * K + lane;
         const Real g = a.csr_g[e];
         lv += g * (ac - ab[nb]);
         cx += 0.5 * (double)g * ((double)ac - (double)ab[nb])
@@ -138,8 +146,11 @@ __device__ __forceinline__ Real mk_adj_couple_graph(
 {
     const Real ac = ab[c], v = V1[c], v0 = V0[c];
     Real lv = (Real)0, id = (Real)0;
-    int e = a.csr_ptr[c];
-    const int e1 = a.csr_ptr[c + 1];
+    /* rows of one copy for a graph of interleaved copies (see mk_graph) */
+    const int K = a.gcopies;
+    const int row = K > 1 ? c / K : c, lane = c - row * K;
+    int e = a.csr_ptr[row];
+    const int e1 = a.csr_ptr[row + 1];
     for (; e + 4 <= e1; e += 4) {
         int t[4];
         Real gg[4], an[4], vn[4], wn[4];
@@ -147,7 +158,7 @@ __device__ __forceinline__ Real mk_adj_couple_graph(
         for (int k = 0; k < 4; ++k) t[k] = a.csr_idx[e + k];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int nb = t[k] < 0 ? ~t[k] : t[k];
+            const int nb = (t[k] < 0 ? ~t[k] : t[k]) * K + lane;
             gg[k] = a.csr_g[e + k];
             an[k] = ab[nb];
             vn[k] = V1[nb];
@@ -164,7 +175,7 @@ __device__ __forceinline__ Real mk_adj_couple_graph(
     }
     for (; e < e1; ++e) {
         const int t = a.csr_idx[e];
-        const int nb = t < 0 ? ~t : t;
+        const int nb = (t < 0 ? ~t : t) * K + lane;
         const Real g = a.csr_g[e];
         lv += g * (ac - ab[nb]);
         cx += 0.5 * (double)g * ((double)ac - (double)ab[nb])
@@ -203,7 +214,8 @@ __device__ __forceinline__ void mk_adj_cell(
     const Real idiff = mk_hetero(V, a.gx_field, a.gy_field, c, ix, iy, nx,
                                  ny);
 #elif MK_COUPLING == 3
-    const Real idiff = mk_graph(V, a.csr_ptr, a.csr_idx, a.csr_g, row, self);
+    const Real idiff = mk_graph(V, a.csr_ptr, a.csr_idx, a.csr_g, row, self,
+                                a.gcopies);
 #else
     const Real idiff = (Real)0;
     (void)V;
@@ -258,7 +270,7 @@ mk_adj_step(const MkArgs a, const MkAdj j, const int s)
 #if MK_COUPLING == 3
             if (!fused)
                 idiff0 = mk_graph(j.yn + MK_IVM * np, a.csr_ptr, a.csr_idx,
-                                  a.csr_g, c, c);
+                                  a.csr_g, c, c, a.gcopies);
             {
                 const MkStep& p = a.steps[s];
                 const Real pace = mk_is_paced(a, c, ix, iy) ? p.pace

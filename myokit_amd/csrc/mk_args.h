@@ -49,6 +49,11 @@ struct MkArgsT {
     int ncols;                  /* per-cell log columns */
     int nan_check;              /* test state 0 of nan_cell at log steps */
     int ndom;                   /* number of domains (block driver) */
+    int gcopies;                /* graph of identical copies interleaved:
+                                   CSR rows are copy positions (device cell
+                                   c is row c / gcopies), neighbours are
+                                   positions (q -> q * gcopies + c %
+                                   gcopies); 1: a row per cell */
     R gx, gy;                   /* homogeneous conductances */
 };
 

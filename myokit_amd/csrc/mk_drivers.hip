@@ -113,14 +113,24 @@ __device__ __forceinline__ Real mk_hetero(
 /* Graph coupling as a gather over the cell's connections, in the order of
  * the connection list: the same additions, in the same order, as the
  * reference's edge scatter (diff_arb_step, openclsim.cl:537-556) when its
- * atomics happen to run in list order. V is indexed by `base + j`. */
+ * atomics happen to run in list order. V is indexed by `base + j`. With
+ * copies > 1 (a graph of identical copies held interleaved, MkArgs.gcopies)
+ * the rows are those of one copy: cell `self` reads row self / copies, and
+ * a neighbour position q is cell q * copies + self % copies. */
 template <typename I>
 __device__ __forceinline__ Real mk_graph(
     const Real* __restrict__ V, const int* __restrict__ ptr,
-    const int* __restrict__ idx, const Real* __restrict__ g, I row, I self)
+    const int* __restrict__ idx, const Real* __restrict__ g, I row, I self,
+    int copies = 1)
 {
     const Real v = V[self];
     Real i = (Real)0;
+    const bool cp = copies > 1;
+    I lane = 0;
+    if (cp) {
+        row = self / copies;
+        lane = self - row * copies;
+    }
     int e = ptr[row];
     const int e1 = ptr[row + 1];
     /* four connections per round: their index, conductance and neighbour
@@ -135,8 +145,9 @@ __device__ __forceinline__ Real mk_graph(
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (e + k < e1) {
+                const int q = j[k] >= 0 ? j[k] : ~j[k];
                 gg[k] = g[e + k];
-                vj[k] = V[j[k] >= 0 ? j[k] : ~j[k]];
+                vj[k] = V[cp ? (I)q * copies + lane : (I)q];
             }
         }
 #pragma unroll
@@ -353,7 +364,7 @@ mk_grid(const MkArgs a, const Real* __restrict__ src, Real* __restrict__ dst,
                                  a.nx, a.ny);
 #elif MK_COUPLING == 3
     Real idiff = mk_graph<long long>(V, a.csr_ptr, a.csr_idx, a.csr_g,
-                                           c, c);
+                                           c, c, a.gcopies);
 #else
     Real idiff = (Real)0;
 #endif

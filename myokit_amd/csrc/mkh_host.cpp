@@ -323,6 +323,11 @@ struct Sim : SimBase {
     std::vector<long long> ci, cj;
     std::vector<double> cg;
     bool have_graph = false;
+    /* graphs that are the caller's identical copies, interleaved (device
+     * cell p * copies + k is copy k's cell p): copies asked for, and the
+     * copies of the CSR in use (1: one row per cell) */
+    long long graph_copies = 1;
+    int csr_copies = 1;
 
     /* options */
     long long batch_steps = 1 << 16;
@@ -672,10 +677,70 @@ struct Sim : SimBase {
                                                    : MKH_DRIVER_GRID;
     }
 
+    /* The CSR of one copy, for a graph of graph_copies identical copies
+     * held interleaved: the connection list is the copies' lists one after
+     * the other, copy k's connection e joining cells (a + k, b + k) with
+     * the conductance of copy 0's connection e (a, b multiples of the
+     * number of copies). Rows are copy 0's cells p = a / copies, entries in
+     * list order with neighbour positions; a kernel maps device cell c to
+     * row c / copies and neighbour q to q * copies + c % copies, so every
+     * cell sums its own connections in its own list order, as with a row
+     * per cell. Returns false (and uploads nothing) if the graph is not
+     * such a set of copies. */
+    bool copies_layout()
+    {
+        const long long K = graph_copies;
+        const long long m = (long long)ci.size();
+        if (K < 2 || n % K || m % K || m == 0 || n / K > INT32_MAX)
+            return false;
+        const long long E = m / K, nc = n / K;
+        for (long long e = 0; e < E; ++e)
+            if (ci[e] % K || cj[e] % K) return false;
+        for (long long k = 1; k < K; ++k)
+            for (long long e = 0; e < E; ++e) {
+                const long long f = k * E + e;
+                if (ci[f] != ci[e] + k || cj[f] != cj[e] + k ||
+                    cg[f] != cg[e])
+                    return false;
+            }
+        std::vector<int> cnt(nc + 1, 0);
+        for (long long e = 0; e < E; ++e) {
+            cnt[ci[e] / K + 1]++;
+            cnt[cj[e] / K + 1]++;
+        }
+        for (long long p = 0; p < nc; ++p) cnt[p + 1] += cnt[p];
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        std::vector<int> hidx(std::max<long long>(2 * E, 1), 0);
+        std::vector<R> hg(std::max<long long>(2 * E, 1), (R)0);
+        for (long long e = 0; e < E; ++e) {
+            const int pa = (int)(ci[e] / K), pb = (int)(cj[e] / K);
+            hidx[fill[pa]] = pb;
+            hg[fill[pa]++] = (R)cg[e];
+            hidx[fill[pb]] = ~pa;
+            hg[fill[pb]++] = (R)cg[e];
+        }
+        if (csr_ptr.alloc(cnt.size()) != hipSuccess ||
+            hipMemcpy(csr_ptr.p, cnt.data(), cnt.size() * 4,
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            csr_idx.alloc(hidx.size()) != hipSuccess ||
+            hipMemcpy(csr_idx.p, hidx.data(), hidx.size() * 4,
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            csr_g.alloc(hg.size()) != hipSuccess ||
+            hipMemcpy(csr_g.p, hg.data(), hg.size() * sizeof(R),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        csr_copies = (int)K;
+        ndom = 1;
+        return true;
+    }
+
     int build_graph_layout(const std::vector<long long>& root,
                            const std::vector<long long>& comp_size)
     {
         const long long m = (long long)ci.size();
+        csr_copies = 1;
         if (driver == MKH_DRIVER_BLOCK) {
             /* Domains = components, cells in increasing id order; CSR rows in
              * domain order with neighbour indices local to the domain. */
@@ -734,6 +799,8 @@ struct Sim : SimBase {
             MKH_HIP(hipMemcpy(dom_off.p, offs.data(),
                               offs.size() * sizeof(long long),
                               hipMemcpyHostToDevice));
+        } else if (copies_layout()) {
+            /* done: one copy's CSR, shared by all copies */
         } else {
             /* Global CSR by cell id, entries in connection-list order. */
             std::vector<int> cnt(n + 1, 0);
@@ -928,6 +995,7 @@ struct Sim : SimBase {
         a.csr_ptr = csr_ptr.p;
         a.csr_idx = csr_idx.p;
         a.csr_g = csr_g.p;
+        a.gcopies = csr_copies;
         a.gx_field = gxf.p;
         a.gy_field = gyf.p;
         a.dom_off = dom_off.p;
@@ -1279,6 +1347,15 @@ struct Sim : SimBase {
                                        (long long)i.size());
             }
             choose_driver();
+        } else if (k == "graph_copies") {
+            if (v < 1) return fail(MKH_E_ARG, "graph_copies must be >= 1");
+            graph_copies = v;
+            if (have_graph) {
+                std::vector<long long> i(ci), jj(cj);
+                std::vector<double> g(cg);
+                return set_connections(i.data(), jj.data(), g.data(),
+                                       (long long)i.size());
+            }
         } else if (k == "adj_release") {
             if (v) adj.release();
         } else if (k == "adj_sheet") {

@@ -260,6 +260,7 @@ class SimulationOpenCL:
         if np.array_equal(perm, np.arange(n)):
             perm = iperm = None
         self._order_cache = (self._graph_version, perm, iperm)
+        self._copies = copies
         self._pcolmaps = {}
         return perm, iperm
 
@@ -365,6 +366,7 @@ class SimulationOpenCL:
         # threads per workgroup; None: chosen per layout (_threads)
         self._block_threads = None
         self._domains_cache = None
+        self._copies = 1
         self._tuning = {}
         self._options = {}
         self.last_run_stats = {}
@@ -1099,6 +1101,10 @@ class SimulationOpenCL:
                     ci = np.ascontiguousarray(iperm[ci])
                     cj = np.ascontiguousarray(iperm[cj])
                 cg = np.ascontiguousarray(a[:, 2], dtype=np.float64)
+                # identical copies held interleaved: the library keeps one
+                # copy's rows when the list is the copies' lists in turn
+                _native.check(L.mkh_sim_set_option(
+                    h, b'graph_copies', int(self._copies)), 'set_option')
                 _native.check(L.mkh_sim_set_connections(
                     h, _native.ptr(ci, ctypes.c_longlong),
                     _native.ptr(cj, ctypes.c_longlong),

@@ -442,3 +442,54 @@ def test_adjoint_graph_cell_order_fd_fp64():
             print('graph r %g cell %d (device %d) dL/dgNa %.10g fd %.10g'
                   % (r, c, iperm[c], g[c], d), flush=True)
             assert rel(g[c], d) < 1e-5
+
+
+def test_adjoint_graph_copies_equal_plain_layout_fp64():
+    # A batch of four identical copies of a 2,500-cell graph: held
+    # interleaved with one copy's connection rows, the loss and every field
+    # gradient must equal, bit for bit, those of the same simulation held
+    # in the plain reverse Cuthill-McKee order with a row per cell (the
+    # same operations in every cell, only where the rows come from
+    # differs)
+    from scipy.spatial import cKDTree
+    from myokit_amd import simulation
+    m, K = 2500, 4
+    rng = np.random.default_rng(23)
+    pts = rng.uniform(0, 1, (m, 2))
+    pairs = cKDTree(pts).query_pairs(np.sqrt(6 / (np.pi * m)),
+                                     output_type='ndarray')
+    g = rng.uniform(5, 12, len(pairs))
+    conns = [(int(i) + k * m, int(j) + k * m, float(w))
+             for k in range(K) for (i, j), w in zip(pairs, g)]
+    left = [int(c) + k * m for k in range(K)
+            for c in np.nonzero(pts[:, 0] < 0.1)[0]]
+    gna = 16 * np.exp(rng.uniform(np.log(.5), np.log(2), m * K))
+    probes = [int(np.argmin(np.abs(pts[:, 0] - 0.4))) + k * m
+              for k in range(K)]
+
+    def run():
+        s = myokit.SimulationOpenCL(
+            myokit.load_model('example'), myokit.pacing.blocktrain(1000, 2),
+            ncells=m * K, precision=64, rl=True)
+        s.set_step_size(0.01)
+        s.set_connections(conns)
+        s.set_paced_cell_list(left)
+        s.set_field('ina.gNa', gna)
+        adj = TissueAdjoint(s, probes, checkpoint_every=100)
+        target = np.full((adj.n_steps(4.0), K), -60.0)
+        loss, grads, _, _ = adj.loss_and_gradient(4.0, target)
+        return s, loss, np.array(grads['ina.gNa'])
+
+    s1, l1, g1 = run()
+    assert s1._copies == K
+    orig = simulation._graph_copies
+    simulation._graph_copies = lambda *a, **k: 1
+    try:
+        s2, l2, g2 = run()
+    finally:
+        simulation._graph_copies = orig
+    assert s2._copies == 1
+    print('graph copies adjoint: loss %.17g vs %.17g, max|dg| %.3g'
+          % (l1, l2, float(np.max(np.abs(g1 - g2)))), flush=True)
+    assert l1 == l2
+    np.testing.assert_array_equal(g1, g2)

@@ -148,15 +148,18 @@ def test_logged_quantities_graph(driver):
     assert_rows_equal('graph ' + driver, P, O, LR_VARS)
 
 
-def test_logged_quantities_graph_copies():
-    # Four identical copies of a 2,500-cell graph (a batch of parameter
-    # sets: different conductances and fields per copy), which the device
-    # holds with the copies interleaved: every logged column in the
-    # caller's order, bit for bit
+@pytest.mark.parametrize('same_g', [True, False])
+def test_logged_quantities_graph_copies(same_g):
+    # Four copies of a 2,500-cell graph (a batch of parameter sets, with
+    # their own fields), which the device holds with the copies
+    # interleaved; with the same conductances in every copy the library
+    # keeps one copy's connection rows for all of them. Every logged column
+    # in the caller's order, bit for bit
     m, K = 2500, 4
     base = _geometric_graph(m, 9)
     rng = np.random.default_rng(12)
-    conns = [(i + k * m, j + k * m, float(rng.uniform(5, 12)))
+    conns = [(i + k * m, j + k * m,
+              g if same_g else float(rng.uniform(5, 12)))
              for k in range(K) for (i, j, g) in base]
     fields = {'ina.gNa': 16 * np.exp(rng.uniform(np.log(.5), np.log(2),
                                                   m * K))}
@@ -166,6 +169,7 @@ def test_logged_quantities_graph_copies():
     s = sc.product(myokit.DOUBLE_PRECISION)
     perm, iperm = s._cell_order()
     assert perm is not None and perm[1] - perm[0] == m  # interleaved
+    assert s._copies == K
     P, O, used = run_logged(sc, 6.0, LR_VARS, 0.5)
     assert used == 'grid'
     assert_rows_equal('graph copies ' + used, P, O, LR_VARS)
