@@ -202,7 +202,12 @@ int mkh_sim_clean(void* sim);
  *   "adj_release"   1: free the adjoint's trajectory and work buffers
  *   "adj_sheet"     0: on a persistent sheet, take the adjoint's reverse
  *                   sweep one launch per step instead of one persistent
- *                   launch per segment (default 1) */
+ *                   launch per segment (default 1)
+ *   "graph_copies"  K: the next graph is K identical copies held
+ *                   interleaved (cell p of copy k is cell p * K + k), the
+ *                   copies' connection lists one after the other; if every
+ *                   copy's list is the first's shifted, with the same
+ *                   conductances, one copy's rows serve all (default 1) */
 int mkh_sim_set_option(void* sim, const char* key, long long value);
 
 /* Kernel timing since the last reset (time_kernels = 1): total milliseconds
