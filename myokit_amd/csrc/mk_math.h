@@ -77,13 +77,21 @@ MK_FN double mk_pow2i(int n)
 
 MK_FN double mk_exp(double x)
 {
-    double k, hi, lo, r, p;
-    int ki;
-    if (x != x) return x;
-    if (x > 0x1.62e42fefa39efp+9) return MK_INF;
-    if (x < -0x1.74910d52d3052p+9) return 0.0;
-    k = MK_FLOOR(x * MK_LOG2E + 0.5);
-    hi = x - k * MK_LN2_HI;
+    /* One straight path for every input, the special cases selected at the
+     * end (no early returns, which a GPU runs as divergent branches in
+     * each of a cell's many exps). The reduction runs on x clamped to the
+     * finite range (NaN to 0), and 2^k is applied as 2^k1 * 2^k2 with
+     * k1 + k2 = k, both normal powers: the product rounds once, as a
+     * single scaling would, so the result equals that of scaling by 2^k
+     * directly, with its overflow and gradual underflow. */
+    double xc, k, hi, lo, r, p, y;
+    int ki, k1;
+    const int nan = x != x;
+    const int big = x > 0x1.62e42fefa39efp+9;
+    const int tiny = x < -0x1.74910d52d3052p+9;
+    xc = (nan || big || tiny) ? 0.0 : x;
+    k = MK_FLOOR(xc * MK_LOG2E + 0.5);
+    hi = xc - k * MK_LN2_HI;
     lo = k * MK_LN2_LO;
     r = hi - lo;
     p = 0x1.6124613a86d09p-33;          /* 1/13! */
@@ -101,9 +109,11 @@ MK_FN double mk_exp(double x)
     p = p * r + 1.0;
     p = p * r + 1.0;
     ki = (int)k;
-    if (ki > 1023) return (p * 2.0) * mk_pow2i(1023);
-    if (ki < -1021) return (p * mk_pow2i(ki + 200)) * 0x1p-200;
-    return p * mk_pow2i(ki);
+    k1 = ki >> 1;
+    y = (p * mk_pow2i(k1)) * mk_pow2i(ki - k1);
+    y = big ? MK_INF : y;
+    y = tiny ? 0.0 : y;
+    return nan ? x : y;
 }
 
 MK_FN double mk_log(double x)
