@@ -470,9 +470,9 @@ def main():
     ap.add_argument('--nx', type=int, default=1024)
     ap.add_argument('--ny', type=int, default=1024)
     ap.add_argument('--precision', type=int, default=32)
-    ap.add_argument('--cpu-sample-ms', type=float, default=5.0,
-                    help='simulated time of the CPU oracle sample (5 ms of '
-                    'the 1024x1024 sheet: ~13 s on 16 host threads)')
+    ap.add_argument('--cpu-sample-ms', type=float, default=10.0,
+                    help='simulated time of the CPU oracle sample (10 ms of '
+                    'the 1024x1024 sheet: ~12 s on 16 host threads)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--adjoint-ms', type=float, default=20.0,
                     help='window of the fwd+adjoint timing (0: skip)')

@@ -870,8 +870,10 @@ def grid_waves(precision):
     on the 1024^2 sheet 1.04e10 cell-steps/s unbounded, 1.11e10 at 5 and
     1.08e10 at 6 (24 spilled), bit-identical. With the branch-free exp the
     step takes 90 registers unbounded (5 waves), 72 at 6 or 7 waves, 64
-    with spills at 8: 1.24e10 at 5, 6 and 7, 1.22e10 at 8. Single
-    precision needs no bound (38 registers, 8 waves)."""
+    with spills at 8: 1.24e10 at 5, 6 and 7, 1.22e10 at 8; with its
+    polynomial in fused multiply-adds, 92 unbounded, 74 at 6: 1.547e10 at
+    5 and 6, 1.535e10 at 7 (spilled). Single precision needs no bound (38
+    registers, 8 waves)."""
     return 5 if precision == DOUBLE_PRECISION else 0
 
 

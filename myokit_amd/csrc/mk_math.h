@@ -2,18 +2,22 @@
  * mk_math.h -- portable double-precision elementary functions.
  *
  * One implementation of exp, log, pow, sin, cos, tan, asin, acos, atan and
- * log10, written only with IEEE-754 +, -, *, / (correctly rounded on every
- * conforming CPU and GPU), floor, fabs and sqrt, and compiled with
- * floating-point contraction disabled. Because every operation is correctly
- * rounded and the operation order is fixed, the HIP kernel (gfx950, built by
- * hiprtc) and the CPU oracle (gcc) produce bit-identical results. That is how
- * the double-precision engine can match its CPU oracle to the last bit.
+ * log10, written only with IEEE-754 +, -, *, /, fused multiply-add (all
+ * correctly rounded on every conforming CPU and GPU: v_fma_f64 on gfx950,
+ * the FMA3 instruction or the C library's fma on the CPU), floor, rint,
+ * fabs and sqrt, and compiled with floating-point contraction disabled, so
+ * that a fused multiply-add appears exactly where one is written. Because
+ * every operation is correctly rounded and the operation order is fixed,
+ * the HIP kernel (gfx950, built by hiprtc) and the CPU oracle (gcc) produce
+ * bit-identical results. That is how the double-precision engine can match
+ * its CPU oracle to the last bit.
  *
  * Methods (textbook, no vendor library constants):
  *   exp  : Cody & Waite (1980) range reduction x = k*ln2 + r, with ln2 split
  *          into a 32-bit head and a tail so that k*LN2_HI is exact, then the
- *          Taylor polynomial of exp(r) to degree 13 (|r| <= ln2/2, truncation
- *          error < 2^-60), then exact scaling by 2^k.
+ *          Taylor polynomial of exp(r) to degree 13 by Horner's rule in
+ *          fused multiply-adds (|r| <= ln2/2, truncation error < 2^-57),
+ *          then exact scaling by 2^k.
  *   log  : x = m * 2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
  *          log(m) = 2*atanh(s) = 2s(1 + s^2/3 + s^4/5 + ... + s^22/23).
  *   pow  : integer exponents |y| <= 64 by binary powering, otherwise
@@ -54,6 +58,8 @@
 #define MK_INF __builtin_inf()
 #define MK_NAN __builtin_nan("")
 #define MK_FLOOR(x) __builtin_floor(x)
+#define MK_RINT(x) __builtin_rint(x)
+#define MK_FMA(a, b, c) __builtin_fma((a), (b), (c))
 #define MK_FABS(x) __builtin_fabs(x)
 #define MK_SQRT(x) __builtin_sqrt(x)
 
@@ -84,30 +90,29 @@ MK_FN double mk_exp(double x)
      * k1 + k2 = k, both normal powers: the product rounds once, as a
      * single scaling would, so the result equals that of scaling by 2^k
      * directly, with its overflow and gradual underflow. */
-    double xc, k, hi, lo, r, p, y;
+    double xc, k, hi, r, p, y;
     int ki, k1;
     const int nan = x != x;
     const int big = x > 0x1.62e42fefa39efp+9;
     const int tiny = x < -0x1.74910d52d3052p+9;
     xc = (nan || big || tiny) ? 0.0 : x;
-    k = MK_FLOOR(xc * MK_LOG2E + 0.5);
-    hi = xc - k * MK_LN2_HI;
-    lo = k * MK_LN2_LO;
-    r = hi - lo;
-    p = 0x1.6124613a86d09p-33;          /* 1/13! */
-    p = p * r + 0x1.1eed8eff8d898p-29;  /* 1/12! */
-    p = p * r + 0x1.ae64567f544e4p-26;  /* 1/11! */
-    p = p * r + 0x1.27e4fb7789f5cp-22;  /* 1/10! */
-    p = p * r + 0x1.71de3a556c734p-19;  /* 1/9!  */
-    p = p * r + 0x1.a01a01a01a01ap-16;  /* 1/8!  */
-    p = p * r + 0x1.a01a01a01a01ap-13;  /* 1/7!  */
-    p = p * r + 0x1.6c16c16c16c17p-10;  /* 1/6!  */
-    p = p * r + 0x1.1111111111111p-7;   /* 1/5!  */
-    p = p * r + 0x1.5555555555555p-5;   /* 1/4!  */
-    p = p * r + 0x1.5555555555555p-3;   /* 1/3!  */
-    p = p * r + 0x1.0000000000000p-1;   /* 1/2!  */
-    p = p * r + 1.0;
-    p = p * r + 1.0;
+    k = MK_RINT(xc * MK_LOG2E);
+    hi = MK_FMA(-k, MK_LN2_HI, xc);     /* exact: k * LN2_HI is exact */
+    r = MK_FMA(-k, MK_LN2_LO, hi);
+    p = 0x1.6124613a86d09p-33;                  /* 1/13! */
+    p = MK_FMA(p, r, 0x1.1eed8eff8d898p-29);    /* 1/12! */
+    p = MK_FMA(p, r, 0x1.ae64567f544e4p-26);    /* 1/11! */
+    p = MK_FMA(p, r, 0x1.27e4fb7789f5cp-22);    /* 1/10! */
+    p = MK_FMA(p, r, 0x1.71de3a556c734p-19);    /* 1/9!  */
+    p = MK_FMA(p, r, 0x1.a01a01a01a01ap-16);    /* 1/8!  */
+    p = MK_FMA(p, r, 0x1.a01a01a01a01ap-13);    /* 1/7!  */
+    p = MK_FMA(p, r, 0x1.6c16c16c16c17p-10);    /* 1/6!  */
+    p = MK_FMA(p, r, 0x1.1111111111111p-7);     /* 1/5!  */
+    p = MK_FMA(p, r, 0x1.5555555555555p-5);     /* 1/4!  */
+    p = MK_FMA(p, r, 0x1.5555555555555p-3);     /* 1/3!  */
+    p = MK_FMA(p, r, 0x1.0000000000000p-1);     /* 1/2!  */
+    p = MK_FMA(p, r, 1.0);
+    p = MK_FMA(p, r, 1.0);
     ki = (int)k;
     k1 = ki >> 1;
     y = (p * mk_pow2i(k1)) * mk_pow2i(ki - k1);

@@ -39,7 +39,7 @@ import subprocess
 
 import numpy as np
 
-from oracle.cgen import BUILD, MATH_H, model_c_parts
+from oracle.cgen import BUILD, MATH_H, cpu_flags, model_c_parts
 
 RTOL = ATOL = 1e-10
 
@@ -85,9 +85,10 @@ def _build(source):
         src = os.path.join(BUILD, 'ode_' + h + '.c')
         with open(src, 'w') as f:
             f.write(source)
-        cmd = ['gcc', '-O2', '-fPIC', '-shared', '-ffp-contract=off',
-               '-fno-fast-math', '-I', os.path.dirname(MATH_H), '-o',
-               so + '.tmp', src, '-lm']
+        cmd = (['gcc', '-O2', '-fPIC', '-shared', '-ffp-contract=off',
+                '-fno-fast-math'] + cpu_flags()
+               + ['-I', os.path.dirname(MATH_H), '-o', so + '.tmp', src,
+                  '-lm'])
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError('ODE build failed:\n' + r.stderr)
