@@ -203,6 +203,9 @@ int mkh_sim_clean(void* sim);
  *   "adj_sheet"     0: on a persistent sheet, take the adjoint's reverse
  *                   sweep one launch per step instead of one persistent
  *                   launch per segment (default 1)
+ *   "adj_overlap"   0: recompute earlier segments in line with the reverse
+ *                   steps, also where a second stream could overlap them
+ *                   (default 1)
  *   "graph_copies"  K: the next graph is K identical copies held
  *                   interleaved (cell p of copy k is cell p * K + k), the
  *                   copies' connection lists one after the other; if every
@@ -245,17 +248,24 @@ int mkh_host_free(void* ptr);
 
 /* Wall time in ms of the last mkh_adj_run's phases: ms3[0] forward with
  * checkpoints, ms3[1] recomputation of earlier segments, ms3[2] reverse
- * steps. No counterpart in the reference. */
+ * steps (with MKH_ADJ_REV_STEP_OVERLAP the recomputation runs during the
+ * reverse steps: ms3[2] is then the whole reverse phase, ms3[1] the part
+ * of it the second stream spent recomputing). No counterpart in the
+ * reference. */
 int mkh_adj_timing(void* sim, double* ms3);
 
 /* The reverse kernel the last mkh_adj_run used: MKH_ADJ_REV_STEP (one
  * launch per time step over all cells), MKH_ADJ_REV_BLOCK (one launch per
- * segment, a workgroup per domain) or MKH_ADJ_REV_SHEET (one cooperative
- * launch per segment over the sheet's tiles), -1 before the first run; and
+ * segment, a workgroup per domain), MKH_ADJ_REV_SHEET (one cooperative
+ * launch per segment over the sheet's tiles) or MKH_ADJ_REV_STEP_OVERLAP
+ * (as MKH_ADJ_REV_STEP, with each earlier segment recomputed on a second
+ * stream while the later one's reverse steps run: small domains, whose
+ * kernels leave most of the device idle), -1 before the first run; and
  * the number of reverse launches it made. No counterpart in the reference. */
 #define MKH_ADJ_REV_STEP 0
 #define MKH_ADJ_REV_BLOCK 1
 #define MKH_ADJ_REV_SHEET 2
+#define MKH_ADJ_REV_STEP_OVERLAP 3
 int mkh_adj_reverse(void* sim, int* kind, long long* launches);
 
 /* The caller's cell order for mkh_adj_run's field gradients: caller cell c

@@ -143,7 +143,10 @@ class TissueAdjoint:
 
     def timing(self):
         """Wall time in ms of the last ``loss_and_gradient``'s phases:
-        ``{'forward', 'recompute', 'reverse'}``."""
+        ``{'forward', 'recompute', 'reverse'}``. With the 'step-overlap'
+        reverse kernel the recomputation runs during the reverse phase:
+        'reverse' is then the whole phase and 'recompute' the part of it
+        the second stream spent recomputing."""
         ms = np.zeros(3)
         eng = getattr(self, '_eng', None)
         if eng is None:
@@ -154,15 +157,18 @@ class TissueAdjoint:
 
     def reverse_kernel(self):
         """The reverse kernel of the last ``loss_and_gradient`` and its
-        number of launches: ``('step' | 'block' | 'sheet', launches)``, or
-        ``(None, 0)`` before the first call."""
+        number of launches: ``('step' | 'step-overlap' | 'block' | 'sheet',
+        launches)``, or ``(None, 0)`` before the first call. 'step-overlap'
+        is the one-step kernel with each earlier segment recomputed on a
+        second stream while the later one's reverse steps run (small
+        domains)."""
         eng = getattr(self, '_eng', None)
         if eng is None:
             return None, 0
         kind, n = ctypes.c_int(-1), ctypes.c_longlong(0)
         _native.check(_native.lib().mkh_adj_reverse(
             eng.h, ctypes.byref(kind), ctypes.byref(n)), 'mkh_adj_reverse')
-        names = {0: 'step', 1: 'block', 2: 'sheet'}
+        names = {0: 'step', 1: 'block', 2: 'sheet', 3: 'step-overlap'}
         return names.get(kind.value), n.value
 
     def release(self):

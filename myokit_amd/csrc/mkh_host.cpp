@@ -363,6 +363,9 @@ struct Sim : SimBase {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         for (auto e : row_ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto e : rc_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (rc_stream) (void)hipStreamDestroy(rc_stream);
         if (copy_stream) (void)hipStreamDestroy(copy_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1360,6 +1363,8 @@ struct Sim : SimBase {
             if (v) adj.release();
         } else if (k == "adj_sheet") {
             no_sheet_rev = v == 0;
+        } else if (k == "adj_overlap") {
+            no_adj_overlap = v == 0;
         } else {
             return fail(MKH_E_ARG, "unknown option " + k);
         }
@@ -1400,10 +1405,16 @@ struct Sim : SimBase {
     /* "adj_sheet" option: 0 keeps the sheet's reverse sweep on the
      * one-step kernel (mk_adj_step) */
     bool no_sheet_rev = false;
+    /* "adj_overlap" option: 0 keeps the recomputation of earlier segments
+     * in line with the reverse steps on small domains */
+    bool no_adj_overlap = false;
+    /* second stream (default priority) for overlapped recomputation */
+    hipStream_t rc_stream = nullptr;
+    hipEvent_t rc_ev[4] = {};
 
     struct AdjBufs {
         DevBuf<Step> steps;
-        DevBuf<R> ckpt, traj, work, work2, mu[2], ab[2];
+        DevBuf<R> ckpt, traj, traj2, work, work2, mu[2], ab[2];
         DevBuf<double> gfield, gx, gy, loss, target, trace;
         DevBuf<int> probe_of;
         DevBuf<long long> probes;
@@ -1412,6 +1423,7 @@ struct Sim : SimBase {
             steps.release();
             ckpt.release();
             traj.release();
+            traj2.release();
             work.release();
             work2.release();
             for (int k = 0; k < 2; ++k) {
@@ -1579,14 +1591,17 @@ struct Sim : SimBase {
 
         Args a = base_args();
         a.steps = dsteps.p;
-        auto grid_step = [&](R* src, R* dst, long long si) -> int {
+        auto grid_step = [&](R* src, R* dst, long long si,
+                             hipStream_t st) -> int {
             int sii = (int)si;
             void* params[] = {&a, &src, &dst, &sii};
             MKH_HIP(hipModuleLaunchKernel(prog->f_grid, ggrid, 1, 1, bt, 1,
-                                          1, 0, stream, params, nullptr));
+                                          1, 0, st, params, nullptr));
             return 0;
         };
-        auto slot = [&](long long k) { return traj.p + (size_t)k * SN; };
+        /* tbase: the trajectory buffer of the segment being reversed */
+        R* tbase = traj.p;
+        auto slot = [&](long long k) { return tbase + (size_t)k * SN; };
         /* The persistent drivers (sheet, block, cells) advance a scratch
          * copy of the state (they read and write state_a) and store each
          * step's state into the trajectory when asked. The block layout
@@ -1614,8 +1629,28 @@ struct Sim : SimBase {
             sheet_rev = (long long)per_cu * prop.multiProcessorCount >=
                         (long long)rtx * rty && !no_sheet_rev;
         }
+        /* Small domains on the one-step kernels: segment sg - 1 is
+         * recomputed on a second stream while the reverse steps of segment
+         * sg run, the two segments' states in two trajectory buffers. Only
+         * where the workgroups of both kernels fit on the device at once
+         * (otherwise each would only slow the other). Same kernels, same
+         * inputs: the results are those of the in-line order. */
+        int ncu = 0;
+        MKH_HIP(hipDeviceGetAttribute(
+            &ncu, hipDeviceAttributeMultiprocessorCount, cfg.device));
+        const bool overlap = !use_pers && nseg > 1 && !no_adj_overlap &&
+                             (long long)ggrid + cgrid <= ncu;
+        if (overlap) {
+            MKH_HIP(adj.traj2.alloc((size_t)tslots * SN));
+            if (!rc_stream) {
+                MKH_HIP(hipStreamCreate(&rc_stream));
+                for (auto& e : rc_ev) MKH_HIP(hipEventCreate(&e));
+            }
+        }
         adj_rev_kind = sheet_rev ? MKH_ADJ_REV_SHEET
-                       : block_rev ? MKH_ADJ_REV_BLOCK : MKH_ADJ_REV_STEP;
+                       : block_rev ? MKH_ADJ_REV_BLOCK
+                       : overlap ? MKH_ADJ_REV_STEP_OVERLAP
+                                 : MKH_ADJ_REV_STEP;
         adj_rev_launches = 0;
         DevBuf<R>& work = adj.work;
         DevBuf<R>& work2 = adj.work2;
@@ -1684,7 +1719,7 @@ struct Sim : SimBase {
                 MKH_HIP(hipMemcpyAsync(ckpt.p + sg * SN, wa, SN * sizeof(R),
                                        hipMemcpyDeviceToDevice, stream));
                 for (long long si = s0; si < s1; ++si) {
-                    int e = grid_step(wa, wb, si);
+                    int e = grid_step(wa, wb, si, stream);
                     if (e) return e;
                     std::swap(wa, wb);
                 }
@@ -1694,7 +1729,8 @@ struct Sim : SimBase {
                     MKH_HIP(hipMemcpyAsync(slot(0), wa, SN * sizeof(R),
                                            hipMemcpyDeviceToDevice, stream));
                 for (long long si = s0; si < s1; ++si) {
-                    int e = grid_step(slot(si - s0), slot(si - s0 + 1), si);
+                    int e = grid_step(slot(si - s0), slot(si - s0 + 1), si,
+                                      stream);
                     if (e) return e;
                 }
             }
@@ -1728,7 +1764,68 @@ struct Sim : SimBase {
             return 0;
         };
         double recompute_ms = 0;
-        for (long long sg = nseg - 1; sg >= 0; --sg) {
+        /* overlapped order: segment sg's states are in bufs[(nseg - 1 - sg)
+         * % 2]; the recomputation of sg - 1 (second stream) waits for the
+         * reverse steps of sg + 1 to release its buffer, the reverse steps
+         * of sg for the recomputation of sg. rc_ev[2 * parity] and
+         * rc_ev[2 * parity + 1] bracket one recomputation, read back
+         * (host-side) before they are recorded again. */
+        R* bufs[2] = {traj.p, adj.traj2.p};
+        hipEvent_t freed = ev[1];       /* forward done: checkpoints ready */
+        auto rc_collect = [&](int par) -> int {
+            MKH_HIP(hipEventSynchronize(rc_ev[2 * par + 1]));
+            float ms = 0;
+            MKH_HIP(hipEventElapsedTime(&ms, rc_ev[2 * par],
+                                        rc_ev[2 * par + 1]));
+            recompute_ms += ms;
+            return 0;
+        };
+        bool rc_pending[2] = {false, false};
+        for (long long sg = nseg - 1; overlap && sg >= 0; --sg) {
+            const long long s0 = sg * Kl, s1 = std::min(N, s0 + Kl);
+            if (sg > 0) {
+                /* recompute segment sg - 1 into the other buffer */
+                const long long p0 = (sg - 1) * Kl;
+                const int par = (int)((nseg - sg) % 2);
+                R* base = bufs[par];
+                if (rc_pending[par]) {
+                    int e = rc_collect(par);
+                    if (e) return e;
+                }
+                MKH_HIP(hipStreamWaitEvent(rc_stream, freed, 0));
+                MKH_HIP(hipEventRecord(rc_ev[2 * par], rc_stream));
+                MKH_HIP(hipMemcpyAsync(base, ckpt.p + (sg - 1) * SN,
+                                       SN * sizeof(R),
+                                       hipMemcpyDeviceToDevice, rc_stream));
+                for (long long si = p0; si < s0; ++si) {
+                    int e = grid_step(base + (size_t)(si - p0) * SN,
+                                      base + (size_t)(si - p0 + 1) * SN, si,
+                                      rc_stream);
+                    if (e) return e;
+                }
+                MKH_HIP(hipEventRecord(rc_ev[2 * par + 1], rc_stream));
+                rc_pending[par] = true;
+            }
+            /* reverse steps of segment sg, once its states are there */
+            const int cpar = (int)((nseg - 1 - sg) % 2);
+            tbase = bufs[cpar];
+            if (sg < nseg - 1)
+                MKH_HIP(hipStreamWaitEvent(stream, rc_ev[2 * cpar + 1], 0));
+            for (long long si = s1 - 1; si >= s0; --si) {
+                int e = reverse_step(slot(si - s0), slot(si - s0 + 1), si,
+                                     (int)si);
+                if (e) return e;
+            }
+            /* its buffer is free for segment sg - 2 after these steps */
+            MKH_HIP(hipEventRecord(ev[2], stream));
+            freed = ev[2];
+        }
+        for (int par = 0; par < 2; ++par)
+            if (rc_pending[par]) {
+                int e = rc_collect(par);
+                if (e) return e;
+            }
+        for (long long sg = nseg - 1; !overlap && sg >= 0; --sg) {
             const long long s0 = sg * Kl, s1 = std::min(N, s0 + Kl);
             if (sg < nseg - 1) {
                 MKH_HIP(hipEventRecord(ev[2], stream));
@@ -1744,7 +1841,7 @@ struct Sim : SimBase {
                 } else {
                     for (long long si = s0; si < s1; ++si) {
                         int e = grid_step(slot(si - s0), slot(si - s0 + 1),
-                                          si);
+                                          si, stream);
                         if (e) return e;
                     }
                 }
@@ -1823,7 +1920,8 @@ struct Sim : SimBase {
             MKH_HIP(hipEventElapsedTime(&f1, ev[1], ev[3]));
             adj_ms[0] = f0;
             adj_ms[1] = recompute_ms;
-            adj_ms[2] = f1 - recompute_ms;
+            /* overlapped: the recomputation ran during the reverse phase */
+            adj_ms[2] = overlap ? f1 : f1 - recompute_ms;
         }
         /* Reductions in double precision, in cell / workgroup order */
         std::vector<double> h(npad == n ? (size_t)npad

@@ -264,7 +264,11 @@ def test_adjoint_sheet_driver_matches_grid_fp64(K):
               'loss', out[-1][0], 'dgx', out[-1][2], 'dgy', out[-1][3],
               flush=True)
         assert s.engine_driver() == ('grid' if force_grid else 'sheet')
-        assert adj.reverse_kernel()[0] == ('step' if force_grid else 'sheet')
+        # with recomputation (K = 70) the 8,192-cell grid path overlaps it
+        # with the reverse steps on a second stream
+        assert adj.reverse_kernel()[0] == (
+            ('step' if K >= 300 else 'step-overlap') if force_grid
+            else 'sheet')
     (l1, g1, x1, y1), (l2, g2, x2, y2) = out
     assert l1 == l2
     np.testing.assert_array_equal(g1['ina.gNa'], g2['ina.gNa'])
@@ -493,3 +497,33 @@ def test_adjoint_graph_copies_equal_plain_layout_fp64():
           % (l1, l2, float(np.max(np.abs(g1 - g2)))), flush=True)
     assert l1 == l2
     np.testing.assert_array_equal(g1, g2)
+
+
+@pytest.mark.parametrize('K', [23, 599])
+def test_adjoint_overlapped_recompute_bit_identical_fp64(K):
+    # On a small domain the grid path recomputes each earlier segment on a
+    # second stream while the later segment's reverse steps run, the two
+    # segments' states in two trajectory buffers. The kernels and their
+    # inputs are those of the in-line order, so loss and gradients must be
+    # identical bit for bit. 600 steps: K = 23 gives 27 segments (the last
+    # one shorter), K = 599 two (the last of one step).
+    res = []
+    for overlap in (1, 0):
+        s = cable(n=200)
+        s.set_engine_option('driver', 2)
+        s.set_engine_option('adj_overlap', overlap)
+        adj = TissueAdjoint(s, [3, 77, 150, 199], checkpoint_every=K)
+        assert adj.n_steps(3.0) == 600
+        target = np.full((600, 4), -60.0)
+        loss, g, dgx, _ = adj.loss_and_gradient(3.0, target)
+        kind, launches = adj.reverse_kernel()
+        print('K', K, 'overlap', overlap, 'reverse', kind, launches, 'loss',
+              loss, 'dgx', dgx, 'phases', adj.timing(), flush=True)
+        assert s.engine_driver() == 'grid'
+        assert kind == ('step-overlap' if overlap else 'step')
+        assert launches == 601
+        res.append((loss, {k: np.array(v) for k, v in g.items()}, dgx))
+    (l1, g1, x1), (l2, g2, x2) = res
+    assert l1 == l2 and x1 == x2
+    for name in ('ina.gNa', 'ica.gCa'):
+        assert np.array_equal(g1[name], g2[name]), name
