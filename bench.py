@@ -370,7 +370,10 @@ def other_workload(args):
             oracle.set_paced_cell_list(list(s._paced_cells))
         if oracle is not None and not args.no_cpu:
             oracle.dt = 0.005
-            sample = {'cable': 20.0, 'cells': 0.5, 'cables': 0.5}[w]
+            # a few seconds of host time each: the whole 1000 ms of the
+            # 200-cell cable, 20 ms of the uncoupled cells, 10 ms of the
+            # cables
+            sample = {'cable': 1000.0, 'cells': 20.0, 'cables': 10.0}[w]
             cv, cs, cel = _cpu_sample(oracle, sample, threads)
             line['cpu_baseline'] = {
                 'value': cv, 'unit': 'cell-steps/s', 'cores': threads,
