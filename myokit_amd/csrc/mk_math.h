@@ -60,6 +60,18 @@
 #define MK_FLOOR(x) __builtin_floor(x)
 #define MK_RINT(x) __builtin_rint(x)
 #define MK_FMA(a, b, c) __builtin_fma((a), (b), (c))
+/* min(max(x, lo), hi) with NaN to lo: v_max_f64 / v_min_f64 on the GPU,
+ * compares on the CPU (where fmax and fmin are library calls) */
+#if defined(__HIPCC_RTC__) || defined(MK_MATH_DEVICE)
+  #define MK_CLAMP(x, lo, hi) __builtin_fmin(__builtin_fmax((x), (lo)), (hi))
+#else
+  static inline double mk_clamp_(double x, double lo, double hi)
+  {
+      x = x >= lo ? x : lo;
+      return x <= hi ? x : hi;
+  }
+  #define MK_CLAMP(x, lo, hi) mk_clamp_((x), (lo), (hi))
+#endif
 #define MK_FABS(x) __builtin_fabs(x)
 #define MK_SQRT(x) __builtin_sqrt(x)
 
@@ -83,19 +95,18 @@ MK_FN double mk_pow2i(int n)
 
 MK_FN double mk_exp(double x)
 {
-    /* One straight path for every input, the special cases selected at the
-     * end (no early returns, which a GPU runs as divergent branches in
-     * each of a cell's many exps). The reduction runs on x clamped to the
-     * finite range (NaN to 0), and 2^k is applied as 2^k1 * 2^k2 with
-     * k1 + k2 = k, both normal powers: the product rounds once, as a
-     * single scaling would, so the result equals that of scaling by 2^k
-     * directly, with its overflow and gradual underflow. */
+    /* One straight path for every input (no early returns, which a GPU
+     * runs as divergent branches in each of a cell's many exps). x is
+     * clamped to [-746, 710] (NaN to -746), where 2^k stays within
+     * 2^-1076 .. 2^1024: below -745.1332 (-1075 ln 2) the result rounds
+     * to 0 and above 709.7827 (ln of the largest double) it overflows to
+     * inf by itself, as exp's do. 2^k is
+     * applied as 2^k1 * 2^k2 with k1 + k2 = k, both normal powers: the
+     * product rounds once, as a single scaling would, with its overflow
+     * and gradual underflow. A NaN argument is returned at the end. */
     double xc, k, hi, r, p, y;
     int ki, k1;
-    const int nan = x != x;
-    const int big = x > 0x1.62e42fefa39efp+9;
-    const int tiny = x < -0x1.74910d52d3052p+9;
-    xc = (nan || big || tiny) ? 0.0 : x;
+    xc = MK_CLAMP(x, -746.0, 710.0);
     k = MK_RINT(xc * MK_LOG2E);
     hi = MK_FMA(-k, MK_LN2_HI, xc);     /* exact: k * LN2_HI is exact */
     r = MK_FMA(-k, MK_LN2_LO, hi);
@@ -116,9 +127,7 @@ MK_FN double mk_exp(double x)
     ki = (int)k;
     k1 = ki >> 1;
     y = (p * mk_pow2i(k1)) * mk_pow2i(ki - k1);
-    y = big ? MK_INF : y;
-    y = tiny ? 0.0 : y;
-    return nan ? x : y;
+    return x != x ? x : y;
 }
 
 MK_FN double mk_log(double x)
