@@ -93,6 +93,21 @@ MK_FN double mk_pow2i(int n)
     return MK_DBL(((mk_u64)(n + 1023)) << 52);
 }
 
+/* p * 2^k rounded once (-1076 <= k <= 1024, p in [0.5, 2)): IEEE scaleB,
+ * v_ldexp_f64 on the GPU; on the CPU (where ldexp is a library call) as
+ * 2^k1 * 2^k2 with k1 + k2 = k, both normal powers, so that only the
+ * second product rounds, into the subnormals or to inf as scaleB does. */
+#if defined(__HIPCC_RTC__) || defined(MK_MATH_DEVICE)
+  #define MK_SCALE2(p, k) __builtin_ldexp((p), (k))
+#else
+  static inline double mk_scale2_(double p, int k)
+  {
+      const int k1 = k >> 1;
+      return (p * mk_pow2i(k1)) * mk_pow2i(k - k1);
+  }
+  #define MK_SCALE2(p, k) mk_scale2_((p), (k))
+#endif
+
 MK_FN double mk_exp(double x)
 {
     /* One straight path for every input (no early returns, which a GPU
@@ -100,12 +115,9 @@ MK_FN double mk_exp(double x)
      * clamped to [-746, 710] (NaN to -746), where 2^k stays within
      * 2^-1076 .. 2^1024: below -745.1332 (-1075 ln 2) the result rounds
      * to 0 and above 709.7827 (ln of the largest double) it overflows to
-     * inf by itself, as exp's do. 2^k is
-     * applied as 2^k1 * 2^k2 with k1 + k2 = k, both normal powers: the
-     * product rounds once, as a single scaling would, with its overflow
-     * and gradual underflow. A NaN argument is returned at the end. */
+     * inf by itself, as exp's do (2^k applied with one rounding,
+     * MK_SCALE2). A NaN argument is returned at the end. */
     double xc, k, hi, r, p, y;
-    int ki, k1;
     xc = MK_CLAMP(x, -746.0, 710.0);
     k = MK_RINT(xc * MK_LOG2E);
     hi = MK_FMA(-k, MK_LN2_HI, xc);     /* exact: k * LN2_HI is exact */
@@ -124,9 +136,7 @@ MK_FN double mk_exp(double x)
     p = MK_FMA(p, r, 0x1.0000000000000p-1);     /* 1/2!  */
     p = MK_FMA(p, r, 1.0);
     p = MK_FMA(p, r, 1.0);
-    ki = (int)k;
-    k1 = ki >> 1;
-    y = (p * mk_pow2i(k1)) * mk_pow2i(ki - k1);
+    y = MK_SCALE2(p, (int)k);
     return x != x ? x : y;
 }
 
