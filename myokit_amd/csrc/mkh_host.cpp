@@ -1418,6 +1418,8 @@ struct Sim : SimBase {
         DevBuf<double> gfield, gx, gy, loss, target, trace;
         DevBuf<int> probe_of;
         DevBuf<long long> probes;
+        /* the probe list probe_of was built for */
+        std::vector<long long> probe_key;
         void release()
         {
             steps.release();
@@ -1438,6 +1440,7 @@ struct Sim : SimBase {
             trace.release();
             probe_of.release();
             probes.release();
+            probe_key.clear();
         }
     } adj;
 
@@ -1569,11 +1572,21 @@ struct Sim : SimBase {
             MKH_HIP(hipMemset(gxa.p, 0, nparts * sizeof(double)));
             MKH_HIP(hipMemset(gya.p, 0, nparts * sizeof(double)));
             MKH_HIP(hipMemset(lossa.p, 0, lossa.n * sizeof(double)));
-            std::vector<int> pof(npad, -1);
-            for (long long k = 0; k < nprobe; ++k) pof[probes[k]] = (int)k;
-            MKH_HIP(probe_of.alloc(npad));
-            MKH_HIP(hipMemcpy(probe_of.p, pof.data(), npad * sizeof(int),
-                              hipMemcpyHostToDevice));
+            /* the cell -> probe map (npad ints: 33 MB for 8.4 M cells) is
+             * built and uploaded again only when the probes change */
+            if (!(probe_of.p && probe_of.n == (size_t)npad &&
+                  adj.probe_key.size() == (size_t)nprobe &&
+                  std::equal(probes, probes + nprobe,
+                             adj.probe_key.begin()))) {
+                adj.probe_key.clear();
+                std::vector<int> pof(npad, -1);
+                for (long long k = 0; k < nprobe; ++k)
+                    pof[probes[k]] = (int)k;
+                MKH_HIP(probe_of.alloc(npad));
+                MKH_HIP(hipMemcpy(probe_of.p, pof.data(), npad * sizeof(int),
+                                  hipMemcpyHostToDevice));
+                adj.probe_key.assign(probes, probes + nprobe);
+            }
             MKH_HIP(dtarget.alloc(std::max<long long>(1, N * nprobe)));
             if (nprobe > 0)
                 MKH_HIP(hipMemcpy(dtarget.p, target,
@@ -1924,8 +1937,6 @@ struct Sim : SimBase {
             adj_ms[2] = overlap ? f1 : f1 - recompute_ms;
         }
         /* Reductions in double precision, in cell / workgroup order */
-        std::vector<double> h(npad == n ? (size_t)npad
-                                        : std::max<size_t>(npad, gfield.n));
         if (gfield_out && cfg.n_field > 0) {
             if (cell_order.p && prog->f_adj_gather) {
                 /* gathered into the caller's order on the device, then
@@ -1953,14 +1964,17 @@ struct Sim : SimBase {
                                   gfield.n * sizeof(double),
                                   hipMemcpyDeviceToHost));
             } else {
-                MKH_HIP(hipMemcpy(h.data(), gfield.p,
+                std::vector<double> hf(gfield.n);
+                MKH_HIP(hipMemcpy(hf.data(), gfield.p,
                                   gfield.n * sizeof(double),
                                   hipMemcpyDeviceToHost));
                 for (int k = 0; k < cfg.n_field; ++k)
                     for (long long c = 0; c < n; ++c)
-                        gfield_out[k * n + c] = h[(size_t)k * npad + c];
+                        gfield_out[k * n + c] = hf[(size_t)k * npad + c];
             }
         }
+        /* the conductance partial sums (one per workgroup or domain) */
+        std::vector<double> h(nparts);
         double sx = 0, sy = 0, sl = 0;
         MKH_HIP(hipMemcpy(h.data(), gxa.p, nparts * sizeof(double),
                           hipMemcpyDeviceToHost));
