@@ -1708,6 +1708,17 @@ struct Sim : SimBase {
                 for (int k = 0; k < 4; ++k) (void)hipEventDestroy(e[k]);
             }
         } evfree{ev};
+        /* whatever way this call ends (an error return included), nothing
+         * it queued on the second stream outlives it (a later call reuses
+         * the trajectory buffers) or the events above (destroyed after
+         * this, being declared before it) */
+        struct RcDrain {
+            hipStream_t st;
+            ~RcDrain()
+            {
+                if (st) (void)hipStreamSynchronize(st);
+            }
+        } rc_drain{overlap ? rc_stream : nullptr};
         MKH_HIP(hipEventRecord(ev[0], stream));
 
         /* Forward */
