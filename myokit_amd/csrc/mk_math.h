@@ -17,7 +17,8 @@
  *          into a 32-bit head and a tail so that k*LN2_HI is exact, then the
  *          Taylor polynomial of exp(r) to degree 13 by Horner's rule in
  *          fused multiply-adds (|r| <= ln2/2, truncation error < 2^-57),
- *          then exact scaling by 2^k.
+ *          then scaling by 2^k with one rounding (exact unless the result
+ *          is subnormal).
  *   log  : x = m * 2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
  *          log(m) = 2*atanh(s) = 2s(1 + s^2/3 + s^4/5 + ... + s^22/23).
  *   pow  : integer exponents |y| <= 64 by binary powering, otherwise
