@@ -774,14 +774,18 @@ _FLOAT_HELPERS = '''
 /* exp(x) in single precision: p = x * log2(e) rounded, e = its rounding
  * error (exact through fma, plus the tail of log2(e)), then
  * exp(x) = 2^p * 2^e ~ 2^p * (1 + e * ln 2). One v_exp_f32 and five FMA-class
- * operations; within 2 ulp (OpenCL allows 3). */
+ * operations; within 2 ulp (OpenCL allows 3). Where 2^p is already 0 or
+ * inf (underflow, overflow, x = -inf or +inf) it is the result: the
+ * correction would turn inf into NaN when e < 0, and e is NaN for an
+ * infinite x. */
 __device__ __forceinline__ float mk_expf(float x)
 {
     const float p = x * 0x1.715476p+0f;
     const float e = __builtin_fmaf(x, 0x1.4ae0c0p-26f,
                                    __builtin_fmaf(x, 0x1.715476p+0f, -p));
     const float r = __builtin_amdgcn_exp2f(p);
-    return __builtin_fmaf(r, e * 0x1.62e430p-1f, r);
+    const float y = __builtin_fmaf(r, e * 0x1.62e430p-1f, r);
+    return (r == 0.0f || r == __builtin_inff()) ? r : y;
 }
 
 /* native_maths: 2^(x * log2(e)) directly, like OpenCL's native_exp */

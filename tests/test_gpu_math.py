@@ -133,3 +133,78 @@ def test_portable_math_device_equals_oracle():
             raise AssertionError('%s differs at cell %d: device %r, oracle '
                                  '%r' % (var, i[1], got[tuple(i)],
                                          ref[tuple(i)]))
+
+
+def _ulps(got, ref):
+    """|got - ref| in units of the single-precision spacing at ref."""
+    ref32 = ref.astype(np.float32)
+    sp = np.spacing(np.abs(ref32)).astype(np.float64)
+    return np.abs(got - ref) / sp
+
+
+def test_single_precision_math_against_numpy():
+    # The single-precision kernel's functions (hardware v_exp_f32 with the
+    # rounding-error correction of mk_expf, and the device library's log,
+    # sqrt, pow, sin, cos, tan, atan, asin, acos) against numpy in double
+    # precision on the same float inputs. Bars: OpenCL's single-precision
+    # bounds, under which the reference's kernels run (exp, log, log10 and
+    # sqrt 3 ulp; sin, cos, asin and acos 4; tan and atan 5; pow 16), where
+    # the result is a normal float, sin, cos and tan also passing within
+    # 2^-24 absolute near their zeros. Overflow to inf, underflow to 0 and
+    # NaN must appear at the same inputs (away from the last ulps of the
+    # range).
+    m = myokit.parse_model(PROBE)
+    f = _inputs()
+    n = N
+    rng = np.random.default_rng(99)
+    xe = f['c.xe'].copy()
+    fin = np.isfinite(xe)
+    xe[fin] = np.clip(xe[fin], -120, 120)
+    xe[:64] = np.linspace(-104, -85, 64)        # into the float subnormals
+    xe[64:128] = np.linspace(85, 89, 64)        # past the float range
+    xb = 10.0 ** rng.uniform(-37, 37, n)
+    xb[:8] = [0.0, -1.0, np.inf, np.nan, 1.0, 2.0, 0.5, 1e-30]
+    xt = f['c.xt']
+    xs = f['c.xs']
+    fields = {'c.xe': xe, 'c.xt': xt, 'c.xb': xb, 'c.xs': xs}
+    s = myokit.SimulationOpenCL(m, myokit.Protocol(), ncells=n,
+                                diffusion=False,
+                                precision=myokit.SINGLE_PRECISION)
+    s.set_step_size(0.01)
+    for name, v in fields.items():
+        s.set_field(name, v)
+    d = s.run_block(0.01, OUT, log_interval=0.01, report_nan=False)
+    x = {k: np.asarray(v, dtype=np.float32).astype(np.float64)
+         for k, v in fields.items()}
+    with np.errstate(all='ignore'):
+        ref = {
+            'c.e': np.exp(x['c.xe']), 'c.l': np.log(x['c.xb']),
+            'c.l10': np.log10(x['c.xb']), 'c.sq': np.sqrt(x['c.xb']),
+            'c.pw': np.power(x['c.xb'], (x['c.xe'] / 256).astype(
+                np.float32).astype(np.float64)),
+            'c.s': np.sin(x['c.xt']), 'c.co': np.cos(x['c.xt']),
+            'c.t': np.tan(x['c.xt']), 'c.at': np.arctan(x['c.xt']),
+            'c.as': np.arcsin(x['c.xs']), 'c.ac': np.arccos(x['c.xs'])}
+    bars = {'c.e': 3, 'c.l': 3, 'c.l10': 3, 'c.sq': 3, 'c.pw': 16,
+            'c.s': 4, 'c.co': 4, 'c.t': 5, 'c.at': 5, 'c.as': 4, 'c.ac': 4}
+    tiny = 2.0 ** -126
+    for var in OUT:
+        got = np.asarray(d[var], dtype=np.float64)[0]
+        r = ref[var]
+        r32 = r.astype(np.float32).astype(np.float64)
+        # special values where the double result is clearly outside the
+        # float range (not within a few ulp of its edges)
+        big = np.abs(r) > 3.0e38
+        small = (np.abs(r) < 1e-39) & np.isfinite(r)
+        nan = np.isnan(r)
+        assert np.array_equal(np.isnan(got), nan), var
+        assert np.all(np.isinf(got[big & (np.abs(r) > 3.5e38)])), var
+        if var in ('c.e', 'c.pw'):
+            assert np.all(got[small & (np.abs(r) < 1e-46)] == 0), var
+        ok = np.isfinite(r32) & ~nan & (np.abs(r32) >= tiny) & ~big
+        u = _ulps(got[ok], r[ok])
+        if var in ('c.s', 'c.co', 'c.t'):
+            u = np.where(np.abs(got[ok] - r[ok]) <= 2.0 ** -24, 0, u)
+        print(var, 'checked', int(ok.sum()), 'max ulp %.2f' % float(
+            np.max(u) if len(u) else 0), flush=True)
+        assert np.max(u) <= bars[var], (var, float(np.max(u)))
