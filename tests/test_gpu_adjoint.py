@@ -229,7 +229,7 @@ def test_torch_autograd_bridge():
     loss = TissueLoss.apply(adj, dur, target, gx, gy, gna, gca)
     loss.backward()
     l2, g, dgx, _ = adj.loss_and_gradient(dur, target.numpy())
-    assert float(loss) == pytest.approx(l2, rel=1e-14)
+    assert float(loss.detach()) == pytest.approx(l2, rel=1e-14)
     np.testing.assert_allclose(gna.grad.numpy(), g['ina.gNa'], rtol=1e-12)
     np.testing.assert_allclose(gca.grad.numpy(), g['ica.gCa'], rtol=1e-12)
     assert float(gx.grad) == pytest.approx(dgx, rel=1e-12)
